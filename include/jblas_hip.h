@@ -1,0 +1,219 @@
+/*
+ * jblas_hip.h -- C ABI of libjblas_hip.so, the MI355X (gfx950) GEMM engine that
+ * sits behind jBLAS.jl's GEMM entry point.
+ *
+ * Reference interface replaced (paths relative to the jBLAS.jl repository):
+ *   jmul!(D::MMatrix{M,P,T}, A::MMatrix{M,N,T}, X::MMatrix{N,P,T}, ...)  src/gemm.jl:244-348
+ *       "calculates D = A * X"                                             src/gemm.jl:225-243
+ *   kernel!(pD, pA, pX, ::Kernel{Mk,Pk,stride_AD,stride_X,N})  D += A*X      src/kernels.jl:239-241
+ *   initkernel!(pD, pA, pX, ::Kernel{...})                     D  = A*X      src/kernels.jl:273-275
+ *   fastmul!(D, A, X) whole-matrix small GEMM                               src/kernels.jl:202-208
+ *   pick_kernel_size / blocking_structure (static tile policy)              src/kernel_structure.jl:76-99,
+ *                                                                           src/memory_management.jl:78-177
+ *   mrandn (random fill)                                                    src/randmat.jl:11-14
+ * The only FFI precedent in the reference is a ccall of a C-ABI void function
+ * taking three host pointers (test/runtests.jl:97-101); this header keeps that
+ * shape (plain pointers, sizes, no runtime types) and adds BLAS arguments.
+ *
+ * Conventions (all entry points):
+ *   - Column-major storage, BLAS argument order and meaning:
+ *       C = alpha * op(A) * op(B) + beta * C,  op(X) = X ('N') or X^T ('T'/'C'),
+ *       op(A) is m x k, op(B) is k x n, C is m x n.
+ *     beta == 0 never reads C; alpha == 0 or k == 0 returns beta*C without
+ *     reading A or B.
+ *   - Return value: 0 on success; -i when the i-th argument is invalid (BLAS
+ *     xerbla numbering, counting from 1 in the order written here); a positive
+ *     JB_ERR_* code for a runtime failure.  jb_last_error() gives a message
+ *     (thread-local).  Validation happens before any HIP call.
+ *   - Functions without the _dev suffix take HOST pointers and are synchronous:
+ *     they stage operands through the library's device workspace (the drop-in
+ *     for jmul!/gemm! on Julia Arrays).  *_dev functions take DEVICE pointers
+ *     and are asynchronous on `stream` (a hipStream_t; NULL = default stream).
+ *   - Calls are serialised inside the library (one mutex); the device workspace
+ *     is owned by the library and released by jb_finalize().
+ */
+#ifndef JBLAS_HIP_H
+#define JBLAS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------- */
+#define JB_OK 0
+#define JB_ERR_HIP 1         /* a HIP runtime call failed (message has details)   */
+#define JB_ERR_ALLOC 2       /* device workspace could not be allocated           */
+#define JB_ERR_NONFINITE 3   /* emulated path: an operand holds Inf or NaN         */
+#define JB_ERR_GRAPH 4       /* hipGraph capture / instantiate / launch failed     */
+#define JB_ERR_STATE 5       /* library not initialised, or a bad handle           */
+#define JB_ERR_UNSUPPORTED 6 /* valid request this build does not implement        */
+
+/* ---- methods (grouped launches and the pullback choose one) ---------------- */
+#define JB_F64_NATIVE 0      /* FP64 MFMA (v_mfma_f64_16x16x4f64)                  */
+#define JB_F64_RECURRENCE 1  /* FP64 VALU, jmul!'s sequential fma order, bit-exact */
+#define JB_F32_NATIVE 2      /* FP32 MFMA (v_mfma_f32_16x16x4f32)                  */
+#define JB_F32_BF16X3 3      /* FP32 via 3 BF16 slices on BF16 MFMA                */
+#define JB_F64_OZAKI_BASE 16 /* JB_F64_OZAKI_BASE + s: split-BF16 DGEMM, s slices  */
+
+/* ---- library state ------------------------------------------------------------ */
+/* Selects the device, queries its properties (replaces the build-time CPU probe
+ * deps/build.jl:1-38 -> src/cpu_info.jl) and loads the autotuning table named by
+ * JB_TUNE_FILE if set.  Called implicitly by the first GEMM call (device 0). */
+int jb_init(int device);
+/* Frees the device workspace and cached graphs. */
+int jb_finalize(void);
+/* Last error message of the calling thread ("" if none). */
+const char* jb_last_error(void);
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int jb_version(void);
+
+/* Device properties gathered by jb_init (the GPU counterpart of cpu_info.jl's
+ * REGISTER_SIZE / REGISTER_COUNT / CACHELINE_SIZE / CACHE_SIZE constants). */
+typedef struct jb_device_info {
+  char name[64];
+  char arch[32];
+  int32_t compute_units;
+  int32_t clock_khz;         /* peak engine clock */
+  int32_t wavefront_size;
+  int32_t lds_bytes_per_cu;
+  int64_t l2_bytes;
+  int64_t hbm_bytes;
+  double peak_fp64_tflops;   /* CU * clock * 128 flop/clk */
+  double peak_bf16_tflops;   /* CU * clock * 4096 flop/clk (dense) */
+  double peak_fp32_tflops;   /* CU * clock * 256 flop/clk */
+} jb_device_info;
+int jb_get_device_info(jb_device_info* out);
+
+/* ---- jmul! drop-in ------------------------------------------------------------ */
+/* D (M x P) = A (M x N) * X (N x P), column-major, leading dimensions M, N, M.
+ * Per element: acc_1 = A[i,1]*X[1,j]; acc_n = fma(A[i,n], X[n,j], acc_{n-1}),
+ * i.e. src/gemm.jl:86 then :165, bit-identical to the reference recurrence,
+ * and (unlike src/gemm.jl:340-345) the fringe rows/columns are written too.
+ * Replaces jmul!(D, A, X) src/gemm.jl:244-348.  Host pointers, synchronous. */
+int jb_jmul(int64_t M, int64_t N, int64_t P, double* D, const double* A, const double* X);
+int jb_jmul_dev(int64_t M, int64_t N, int64_t P, double* D, const double* A, const double* X,
+                void* stream);
+
+/* ---- BLAS GEMM (gemm! in Julia: LinearAlgebra.BLAS.gemm!(tA,tB,alpha,A,B,beta,C)) */
+/* FP64 on FP64 MFMA.  jmul!(D,A,X) == jb_dgemm('N','N',M,P,N,1,A,M,X,N,0,D,M)
+ * up to the summation order of the matrix core. */
+int jb_dgemm(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
+             const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+             int64_t ldc);
+int jb_dgemm_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
+                 const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                 double* C, int64_t ldc, void* stream);
+
+/* FP64 with the reference's sequential-fma order per element (bit-exact vs the
+ * jmul! recurrence under BLAS alpha/beta/op semantics; kernel!/initkernel!
+ * src/kernels.jl:212-275 with arbitrary leading dimensions). */
+int jb_dgemm_recurrence(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
+                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                        double* C, int64_t ldc);
+int jb_dgemm_recurrence_dev(char transA, char transB, int64_t m, int64_t n, int64_t k,
+                            double alpha, const double* A, int64_t lda, const double* B,
+                            int64_t ldb, double beta, double* C, int64_t ldc, void* stream);
+
+/* Emulated FP64: operands split into `nslices` (1..8) exponent-scaled BF16
+ * slices, slice pairs (t+u <= nslices+1) multiplied on BF16 MFMA with exact
+ * FP32 accumulation, level sums combined in compensated FP64.  The 15th
+ * argument (nslices) is validated as argument 14 in the _dev form too. */
+int jb_dgemm_ozaki(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
+                   const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                   double* C, int64_t ldc, int nslices);
+int jb_dgemm_ozaki_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
+                       const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                       double* C, int64_t ldc, int nslices, void* stream);
+
+/* FP32 on FP32 MFMA. */
+int jb_sgemm(char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha,
+             const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
+             int64_t ldc);
+int jb_sgemm_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha,
+                 const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
+                 int64_t ldc, void* stream);
+
+/* FP32 emulated with 3 BF16 slices per element (hi, mid, lo) on BF16 MFMA and
+ * FP32 accumulation; nprod in {3, 6, 9} slice products (6 = all pairs t+u <= 4). */
+int jb_sgemm_bf16x3(char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha,
+                    const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
+                    float* C, int64_t ldc, int nprod);
+int jb_sgemm_bf16x3_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha,
+                        const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
+                        float* C, int64_t ldc, int nprod, void* stream);
+
+/* ---- batched / grouped GEMM ---------------------------------------------------- */
+/* One problem of a grouped launch.  A, B, C are DEVICE pointers of the method's
+ * element type (double for F64 methods, float for F32 methods). */
+typedef struct jb_gemm_desc {
+  char transA;
+  char transB;
+  char pad_[6];
+  int64_t m, n, k;
+  double alpha, beta;
+  const void* A;
+  int64_t lda;
+  const void* B;
+  int64_t ldb;
+  void* C;
+  int64_t ldc;
+} jb_gemm_desc;
+
+/* All `count` problems in one persistent launch (tile shape and split-K per
+ * problem from the autotuning table).  `descs` is a HOST array.
+ * Methods: JB_F64_NATIVE, JB_F32_BF16X3, JB_F32_NATIVE. */
+int jb_gemm_grouped_dev(int method, int64_t count, const jb_gemm_desc* descs, void* stream);
+
+/* A grouped plan: descriptors uploaded once, the persistent launch captured in
+ * a hipGraph; jb_grouped_plan_launch replays it on `stream`. */
+typedef struct jb_grouped_plan jb_grouped_plan;
+int jb_grouped_plan_create(int method, int64_t count, const jb_gemm_desc* descs, void* stream,
+                           jb_grouped_plan** plan);
+int jb_grouped_plan_launch(jb_grouped_plan* plan, void* stream);
+int jb_grouped_plan_destroy(jb_grouped_plan* plan);
+
+/* ---- autotuner ------------------------------------------------------------------ */
+/* Times every candidate (tile shape x split-K) for each size bucket present in
+ * `descs` and stores the fastest in the in-memory table (replaces the static
+ * pick_kernel_size / blocking_structure choice).  `reps` timed replays each. */
+int jb_autotune_grouped(int method, int64_t count, const jb_gemm_desc* descs, int reps,
+                        void* stream);
+/* Persist / restore the table as JSON (keyed by device, arch, method, bucket). */
+int jb_autotune_save(const char* path);
+int jb_autotune_load(const char* path);
+/* Query the choice for one problem shape: tile id and split-K. */
+int jb_autotune_lookup(int method, int64_t m, int64_t n, int64_t k, int* tile, int* splitk);
+/* Clear the table (static defaults apply again). */
+int jb_autotune_reset(void);
+
+/* ---- differentiation (ChainRulesCore.rrule of gemm!) ---------------------------- */
+/* For C = alpha*op(A)*op(B) + beta*C_in and upstream gradient dC (m x n):
+ *   dA (shape of A) = alpha * dC * op(B)^T   (transA 'N')  or  alpha * op(B) * dC^T ('T')
+ *   dB (shape of B) = alpha * op(A)^T * dC   (transB 'N')  or  alpha * dC^T * op(A) ('T')
+ * Both adjoint GEMMs run in one grouped launch (transposed-layout loaders).
+ * dA or dB may be NULL to skip it.  method: JB_F64_NATIVE, JB_F32_NATIVE, JB_F32_BF16X3. */
+int jb_gemm_pullback_dev(int method, char transA, char transB, int64_t m, int64_t n, int64_t k,
+                         double alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+                         const void* dC, int64_t lddc, void* dA, int64_t ldda, void* dB,
+                         int64_t lddb, void* stream);
+int jb_gemm_pullback(int method, char transA, char transB, int64_t m, int64_t n, int64_t k,
+                     double alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+                     const void* dC, int64_t lddc, void* dA, int64_t ldda, void* dB,
+                     int64_t lddb);
+
+/* ---- seeded device generator (replaces mrandn, src/randmat.jl:11-14) ------------- */
+/* out[i] = 2*((x_i >> 11) * 2^-53) - 1, x_i = SplitMix64(seed) output i; bit-identical
+ * to the host generator. */
+int jb_fill_uniform_dev(uint64_t seed, int64_t n, double* out, void* stream);
+/* out[i] = u_i * 2^e_i, u from stream index 2i, e = emin + floor((emax-emin+1) v), v
+ * from index 2i+1. */
+int jb_fill_wide_dev(uint64_t seed, int64_t n, int emin, int emax, double* out, void* stream);
+/* float32 variant of the uniform fill (u rounded to float). */
+int jb_fill_uniform_f32_dev(uint64_t seed, int64_t n, float* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JBLAS_HIP_H */

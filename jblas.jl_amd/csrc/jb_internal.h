@@ -1,0 +1,69 @@
+// jb_internal.h -- shared declarations of the HIP kernels' host launchers and
+// the runtime (not part of the public C ABI; see include/jblas_hip.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace jb {
+
+// Element type and transposition of one GEMM operand.
+struct GemmArgs {
+  bool ta, tb;          // op(A) = A^T, op(B) = B^T
+  int64_t m, n, k;
+  double alpha, beta;
+  const void* A;
+  int64_t lda;
+  const void* B;
+  int64_t ldb;
+  void* C;
+  int64_t ldc;
+};
+
+// ---- FP64 -----------------------------------------------------------------
+// Native FP64 MFMA GEMM (128x128 block tile, LDS double buffer).
+hipError_t launch_dgemm_mfma(const GemmArgs& g, hipStream_t s);
+// Sequential-fma FP64 GEMM (bit-exact jmul! recurrence).
+hipError_t launch_dgemm_recurrence(const GemmArgs& g, hipStream_t s);
+// C = beta*C (alpha == 0 or k == 0 quick path); beta == 0 writes zeros.
+hipError_t launch_scale_f64(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipStream_t s);
+hipError_t launch_scale_f32(int64_t m, int64_t n, float beta, float* C, int64_t ldc, hipStream_t s);
+
+// ---- split-BF16 emulated FP64 --------------------------------------------
+// Workspace bytes needed for an m x n x k problem with s slices.
+size_t ozaki_workspace_bytes(int64_t m, int64_t n, int64_t k, int s);
+// Split + slice-pair GEMM + compensated combine.  `ws` is device memory of
+// ozaki_workspace_bytes(); `flag` is a device int that the split kernels set
+// to 1 on a non-finite input (caller zeroes it first).
+hipError_t launch_dgemm_ozaki(const GemmArgs& g, int s, void* ws, int* flag, hipStream_t st);
+
+// ---- FP32 -----------------------------------------------------------------
+hipError_t launch_sgemm_mfma(const GemmArgs& g, hipStream_t s);
+size_t bf16x3_workspace_bytes(int64_t m, int64_t n, int64_t k);
+hipError_t launch_sgemm_bf16x3(const GemmArgs& g, int nprod, void* ws, hipStream_t s);
+
+// ---- generator ------------------------------------------------------------
+hipError_t launch_fill_uniform(uint64_t seed, int64_t n, double* out, hipStream_t s);
+hipError_t launch_fill_uniform_f32(uint64_t seed, int64_t n, float* out, hipStream_t s);
+hipError_t launch_fill_wide(uint64_t seed, int64_t n, int emin, int emax, double* out, hipStream_t s);
+
+// XCD-aware block swizzle: hardware places workgroup w on XCD (w % nxcd).
+// Returns a tile id such that each XCD owns a contiguous range of tile ids.
+__device__ __forceinline__ int xcd_remap(int w, int nblocks, int nxcd) {
+  const int q = nblocks / nxcd, r = nblocks % nxcd;
+  const int x = w % nxcd, local = w / nxcd;
+  return (x < r) ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
+}
+
+// Grouped ordering of an (tm x tn) tile grid: GROUP tile-rows walked column by
+// column so that concurrently running tiles share A row panels and B column panels.
+__device__ __forceinline__ void tile_coords(int tile, int tiles_m, int tiles_n, int group, int& tm, int& tn) {
+  const int per_group = group * tiles_n;
+  const int gid = tile / per_group;
+  const int first_m = gid * group;
+  const int gsize = min(tiles_m - first_m, group);
+  const int in_group = tile - gid * per_group;
+  tm = first_m + in_group % gsize;
+  tn = in_group / gsize;
+}
+
+}  // namespace jb

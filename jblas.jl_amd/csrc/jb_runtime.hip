@@ -1,0 +1,496 @@
+// jb_runtime.hip -- host runtime behind include/jblas_hip.h.
+//
+// Responsibilities: BLAS-style argument validation (before any HIP call),
+// library state and device-property query (the GPU replacement of the
+// build-time CPU probe deps/build.jl:1-38 -> src/cpu_info.jl), a stream-ordered
+// device workspace pool, host<->device staging for the host-pointer entry
+// points, and dispatch to the kernels' launchers.
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/jblas_hip.h"
+#include "jb_internal.h"
+#include "jb_runtime.h"
+
+namespace jb {
+
+Runtime& rt() {
+  static Runtime r;
+  return r;
+}
+
+static thread_local std::string tl_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  tl_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(JB_ERR_HIP, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
+
+static bool valid_trans(char t) { return t == 'N' || t == 'n' || t == 'T' || t == 't' || t == 'C' || t == 'c'; }
+static bool is_trans(char t) { return !(t == 'N' || t == 'n'); }
+
+int validate_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, bool alpha_zero, const void* A, int64_t lda,
+                  const void* B, int64_t ldb, const void* C, int64_t ldc, GemmArgs* out) {
+  tl_error.clear();
+  if (!valid_trans(ta)) return fail(-1, "transA must be 'N', 'T' or 'C' (got %d)", (int)ta);
+  if (!valid_trans(tb)) return fail(-2, "transB must be 'N', 'T' or 'C' (got %d)", (int)tb);
+  if (m < 0) return fail(-3, "m < 0");
+  if (n < 0) return fail(-4, "n < 0");
+  if (k < 0) return fail(-5, "k < 0");
+  const int64_t arows = is_trans(ta) ? k : m, brows = is_trans(tb) ? n : k;
+  const bool uses_ab = m > 0 && n > 0 && k > 0 && !alpha_zero;
+  if (uses_ab && A == nullptr) return fail(-7, "A is NULL");
+  if (lda < (arows > 1 ? arows : 1)) return fail(-8, "lda < max(1, %lld)", (long long)arows);
+  if (uses_ab && B == nullptr) return fail(-9, "B is NULL");
+  if (ldb < (brows > 1 ? brows : 1)) return fail(-10, "ldb < max(1, %lld)", (long long)brows);
+  if (m > 0 && n > 0 && C == nullptr) return fail(-12, "C is NULL");
+  if (ldc < (m > 1 ? m : 1)) return fail(-13, "ldc < max(1, %lld)", (long long)m);
+  // Tile-grid sizes are int32 inside the kernels.
+  if (m > (int64_t)1 << 30 || n > (int64_t)1 << 30) return fail(-3, "m or n above 2^30 is not supported");
+  out->ta = is_trans(ta);
+  out->tb = is_trans(tb);
+  out->m = m;
+  out->n = n;
+  out->k = k;
+  out->A = A;
+  out->lda = lda;
+  out->B = B;
+  out->ldb = ldb;
+  out->C = const_cast<void*>(C);
+  out->ldc = ldc;
+  return JB_OK;
+}
+
+int ensure_init() {
+  if (rt().inited) return JB_OK;
+  return init_locked(0);
+}
+
+int init_locked(int device) {
+  Runtime& r = rt();
+  if (r.inited && r.device == device) return JB_OK;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return fail(JB_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= count) return fail(-1, "device %d out of range [0,%d)", device, count);
+  if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+  hipDeviceProp_t p;
+  if ((e = hipGetDeviceProperties(&p, device)) != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+  jb_device_info& d = r.info;
+  memset(&d, 0, sizeof d);
+  snprintf(d.name, sizeof d.name, "%s", p.name);
+  snprintf(d.arch, sizeof d.arch, "%s", p.gcnArchName);
+  d.compute_units = p.multiProcessorCount;
+  d.clock_khz = p.clockRate;
+  d.wavefront_size = p.warpSize;
+  d.lds_bytes_per_cu = (int32_t)p.maxSharedMemoryPerMultiProcessor;
+  d.l2_bytes = p.l2CacheSize;
+  d.hbm_bytes = (int64_t)p.totalGlobalMem;
+  const double cu_ghz = (double)p.multiProcessorCount * (double)p.clockRate * 1e-6;  // CU * GHz
+  d.peak_fp64_tflops = cu_ghz * 128.0 * 1e-3;
+  d.peak_fp32_tflops = cu_ghz * 256.0 * 1e-3;
+  d.peak_bf16_tflops = cu_ghz * 4096.0 * 1e-3;
+  if (r.stream == nullptr) {
+    if ((e = hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking)) != hipSuccess)
+      return hip_fail(e, "hipStreamCreate");
+  }
+  if (r.ws_event == nullptr) {
+    if ((e = hipEventCreateWithFlags(&r.ws_event, hipEventDisableTiming)) != hipSuccess)
+      return hip_fail(e, "hipEventCreate");
+  }
+  r.device = device;
+  r.inited = true;
+  const char* tf = getenv("JB_TUNE_FILE");
+  if (tf && *tf) tune_load_locked(tf);  // a missing or corrupt table leaves the static defaults
+  return JB_OK;
+}
+
+// ---- device buffers --------------------------------------------------------
+int Buffer::reserve(size_t bytes, const char* what) {
+  if (bytes <= size) return JB_OK;
+  const char* fault = getenv("JB_FAULT_ALLOC");
+  if (fault && *fault == '1') return fail(JB_ERR_ALLOC, "%s: allocation of %zu bytes refused (JB_FAULT_ALLOC)", what, bytes);
+  if (ptr) {
+    hipDeviceSynchronize();  // earlier work may still use the old buffer
+    hipFree(ptr);
+    ptr = nullptr;
+    size = 0;
+  }
+  size_t want = bytes + (bytes >> 3);  // grow with headroom
+  hipError_t e = hipMalloc(&ptr, want);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    hipGetLastError();
+    return fail(JB_ERR_ALLOC, "%s: hipMalloc(%zu) failed: %s", what, want, hipGetErrorString(e));
+  }
+  size = want;
+  return JB_OK;
+}
+
+void Buffer::release() {
+  if (ptr) hipFree(ptr);
+  ptr = nullptr;
+  size = 0;
+}
+
+// Stream-ordered use of the shared workspace: a call waits for the previous
+// user of the workspace (on any stream) before touching it.
+int ws_acquire(size_t bytes, hipStream_t s, void** out) {
+  Runtime& r = rt();
+  int rc = r.ws.reserve(bytes, "workspace");
+  if (rc) return rc;
+  if (r.ws_pending) {
+    hipError_t e = hipStreamWaitEvent(s, r.ws_event, 0);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
+  }
+  *out = r.ws.ptr;
+  return JB_OK;
+}
+
+int ws_release(hipStream_t s) {
+  Runtime& r = rt();
+  hipError_t e = hipEventRecord(r.ws_event, s);
+  if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+  r.ws_pending = true;
+  return JB_OK;
+}
+
+// ---- host staging ----------------------------------------------------------
+// Copies a rows x cols column-major host matrix (leading dim ld) to device
+// memory with the same leading dimension.
+static hipError_t h2d(void* dst, const void* src, int64_t rows, int64_t cols, int64_t ld, size_t elt, hipStream_t s) {
+  if (rows == 0 || cols == 0) return hipSuccess;
+  return hipMemcpy2DAsync(dst, ld * elt, src, ld * elt, rows * elt, cols, hipMemcpyHostToDevice, s);
+}
+static hipError_t d2h(void* dst, const void* src, int64_t rows, int64_t cols, int64_t ld, size_t elt, hipStream_t s) {
+  if (rows == 0 || cols == 0) return hipSuccess;
+  return hipMemcpy2DAsync(dst, ld * elt, src, ld * elt, rows * elt, cols, hipMemcpyDeviceToHost, s);
+}
+static size_t span_bytes(int64_t rows, int64_t cols, int64_t ld, size_t elt) {
+  if (rows == 0 || cols == 0) return 0;
+  return (size_t)(ld * (cols - 1) + rows) * elt;
+}
+static size_t round_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Runs `dev` (a device-pointer GEMM) on staged copies of host operands.
+int run_staged(const GemmArgs& g, size_t elt, bool read_c, const std::function<int(const GemmArgs&, hipStream_t)>& dev) {
+  Runtime& r = rt();
+  const int64_t arows = g.ta ? g.k : g.m, acols = g.ta ? g.m : g.k;
+  const int64_t brows = g.tb ? g.n : g.k, bcols = g.tb ? g.k : g.n;
+  const size_t sa = round_up(span_bytes(arows, acols, g.lda, elt));
+  const size_t sb = round_up(span_bytes(brows, bcols, g.ldb, elt));
+  const size_t sc = round_up(span_bytes(g.m, g.n, g.ldc, elt));
+  int rc = r.staging.reserve(sa + sb + sc + 256, "staging");
+  if (rc) return rc;
+  char* base = static_cast<char*>(r.staging.ptr);
+  GemmArgs d = g;
+  d.A = base;
+  d.B = base + sa;
+  d.C = base + sa + sb;
+  hipStream_t s = r.stream;
+  hipError_t e;
+  if ((e = h2d(base, g.A, arows, acols, g.lda, elt, s)) != hipSuccess) return hip_fail(e, "copy A to device");
+  if ((e = h2d(base + sa, g.B, brows, bcols, g.ldb, elt, s)) != hipSuccess) return hip_fail(e, "copy B to device");
+  if (read_c && (e = h2d(d.C, g.C, g.m, g.n, g.ldc, elt, s)) != hipSuccess) return hip_fail(e, "copy C to device");
+  rc = dev(d, s);
+  if (rc) {
+    hipStreamSynchronize(s);
+    return rc;
+  }
+  if ((e = d2h(g.C, d.C, g.m, g.n, g.ldc, elt, s)) != hipSuccess) return hip_fail(e, "copy C to host");
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  return JB_OK;
+}
+
+void finalize_locked() {
+  Runtime& r = rt();
+  if (!r.inited) return;
+  hipDeviceSynchronize();
+  grouped_release_all();
+  r.ws.release();
+  r.staging.release();
+  if (r.flag) {
+    hipFree(r.flag);
+    r.flag = nullptr;
+  }
+  if (r.ws_event) hipEventDestroy(r.ws_event);
+  r.ws_event = nullptr;
+  r.ws_pending = false;
+  if (r.stream) hipStreamDestroy(r.stream);
+  r.stream = nullptr;
+  r.inited = false;
+}
+
+int device_flag(int** out) {
+  Runtime& r = rt();
+  if (!r.flag) {
+    hipError_t e = hipMalloc(&r.flag, 256);
+    if (e != hipSuccess) {
+      r.flag = nullptr;
+      return fail(JB_ERR_ALLOC, "flag allocation failed: %s", hipGetErrorString(e));
+    }
+  }
+  *out = r.flag;
+  return JB_OK;
+}
+
+}  // namespace jb
+
+using namespace jb;
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int jb_version(void) { return 10000; }
+
+const char* jb_last_error(void) { return tl_error.c_str(); }
+
+int jb_init(int device) {
+  std::lock_guard<std::mutex> lk(rt().mu);
+  tl_error.clear();
+  if (rt().inited && rt().device != device) finalize_locked();
+  return init_locked(device);
+}
+
+int jb_finalize(void) {
+  std::lock_guard<std::mutex> lk(rt().mu);
+  finalize_locked();
+  return JB_OK;
+}
+
+int jb_get_device_info(jb_device_info* out) {
+  if (!out) return fail(-1, "out is NULL");
+  std::lock_guard<std::mutex> lk(rt().mu);
+  int rc = ensure_init();
+  if (rc) return rc;
+  *out = rt().info;
+  return JB_OK;
+}
+
+// ---- FP64 native / recurrence ---------------------------------------------
+static int dgemm_dev_locked(const GemmArgs& g, bool recurrence, hipStream_t s) {
+  if (g.m == 0 || g.n == 0) return JB_OK;
+  hipError_t e;
+  if (g.alpha == 0.0 || g.k == 0) {
+    e = launch_scale_f64(g.m, g.n, g.beta, static_cast<double*>(g.C), g.ldc, s);
+  } else {
+    e = recurrence ? launch_dgemm_recurrence(g, s) : launch_dgemm_mfma(g, s);
+  }
+  return e == hipSuccess ? JB_OK : hip_fail(e, recurrence ? "dgemm_recurrence launch" : "dgemm_mfma launch");
+}
+
+#define JB_GEMM_PROLOGUE(ALPHA, BETA)                                                                   \
+  GemmArgs g;                                                                                            \
+  int rc = validate_gemm(transA, transB, m, n, k, (ALPHA) == 0, A, lda, B, ldb, C, ldc, &g);            \
+  if (rc) return rc;                                                                                     \
+  g.alpha = (double)(ALPHA);                                                                             \
+  g.beta = (double)(BETA);                                                                               \
+  std::lock_guard<std::mutex> lk(rt().mu);                                                              \
+  if (g.m == 0 || g.n == 0) return JB_OK;                                                                \
+  rc = ensure_init();                                                                                    \
+  if (rc) return rc;
+
+int jb_dgemm_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                 int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, void* stream) {
+  JB_GEMM_PROLOGUE(alpha, beta)
+  return dgemm_dev_locked(g, false, (hipStream_t)stream);
+}
+
+int jb_dgemm(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+             const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
+  JB_GEMM_PROLOGUE(alpha, beta)
+  return run_staged(g, sizeof(double), beta != 0.0, [](const GemmArgs& d, hipStream_t s) {
+    return dgemm_dev_locked(d, false, s);
+  });
+}
+
+int jb_dgemm_recurrence_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
+                            const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                            int64_t ldc, void* stream) {
+  JB_GEMM_PROLOGUE(alpha, beta)
+  return dgemm_dev_locked(g, true, (hipStream_t)stream);
+}
+
+int jb_dgemm_recurrence(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                        int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
+  JB_GEMM_PROLOGUE(alpha, beta)
+  return run_staged(g, sizeof(double), beta != 0.0, [](const GemmArgs& d, hipStream_t s) {
+    return dgemm_dev_locked(d, true, s);
+  });
+}
+
+static int jmul_validate(int64_t M, int64_t N, int64_t P, const void* D, const void* A, const void* X) {
+  tl_error.clear();
+  if (M < 0) return fail(-1, "M < 0");
+  if (N < 1) return fail(-2, "N < 1 (jmul! needs at least one k-step, src/gemm.jl:86)");
+  if (P < 0) return fail(-3, "P < 0");
+  if (M > 0 && P > 0 && D == nullptr) return fail(-4, "D is NULL");
+  if (M > 0 && A == nullptr) return fail(-5, "A is NULL");
+  if (P > 0 && X == nullptr) return fail(-6, "X is NULL");
+  return JB_OK;
+}
+
+int jb_jmul_dev(int64_t M, int64_t N, int64_t P, double* D, const double* A, const double* X, void* stream) {
+  int rc = jmul_validate(M, N, P, D, A, X);
+  if (rc) return rc;
+  return jb_dgemm_recurrence_dev('N', 'N', M, P, N, 1.0, A, M > 1 ? M : 1, X, N, 0.0, D, M > 1 ? M : 1, stream);
+}
+
+int jb_jmul(int64_t M, int64_t N, int64_t P, double* D, const double* A, const double* X) {
+  int rc = jmul_validate(M, N, P, D, A, X);
+  if (rc) return rc;
+  return jb_dgemm_recurrence('N', 'N', M, P, N, 1.0, A, M > 1 ? M : 1, X, N, 0.0, D, M > 1 ? M : 1);
+}
+
+// ---- FP64 emulated (split-BF16) -------------------------------------------
+static int ozaki_dev_locked(const GemmArgs& g, int s, hipStream_t st) {
+  if (g.m == 0 || g.n == 0) return JB_OK;
+  if (g.alpha == 0.0 || g.k == 0) {
+    hipError_t e = launch_scale_f64(g.m, g.n, g.beta, static_cast<double*>(g.C), g.ldc, st);
+    return e == hipSuccess ? JB_OK : hip_fail(e, "scale launch");
+  }
+  int* flag = nullptr;
+  int rc = device_flag(&flag);
+  if (rc) return rc;
+  void* ws = nullptr;
+  rc = ws_acquire(ozaki_workspace_bytes(g.m, g.n, g.k, s), st, &ws);
+  if (rc) return rc;
+  hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), st);
+  if (e == hipSuccess) e = launch_dgemm_ozaki(g, s, ws, flag, st);
+  int relrc = ws_release(st);
+  if (e != hipSuccess) return hip_fail(e, "dgemm_ozaki launch");
+  if (relrc) return relrc;
+  // The non-finite check needs the flag on the host: this is the one
+  // synchronisation point of the emulated path.
+  int h = 0;
+  e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "non-finite flag readback");
+  if (h) return fail(JB_ERR_NONFINITE, "an operand of the emulated DGEMM holds Inf or NaN; C is unspecified");
+  return JB_OK;
+}
+
+int jb_dgemm_ozaki_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                       int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int nslices,
+                       void* stream) {
+  if (nslices < 1 || nslices > 8) return fail(-14, "nslices must be in 1..8 (got %d)", nslices);
+  JB_GEMM_PROLOGUE(alpha, beta)
+  return ozaki_dev_locked(g, nslices, (hipStream_t)stream);
+}
+
+int jb_dgemm_ozaki(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                   int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int nslices) {
+  if (nslices < 1 || nslices > 8) return fail(-14, "nslices must be in 1..8 (got %d)", nslices);
+  JB_GEMM_PROLOGUE(alpha, beta)
+  return run_staged(g, sizeof(double), beta != 0.0, [nslices](const GemmArgs& d, hipStream_t s) {
+    return ozaki_dev_locked(d, nslices, s);
+  });
+}
+
+// ---- FP32 -------------------------------------------------------------------
+static int sgemm_dev_locked(const GemmArgs& g, int method, int nprod, hipStream_t s) {
+  if (g.m == 0 || g.n == 0) return JB_OK;
+  hipError_t e;
+  if (g.alpha == 0.0 || g.k == 0) {
+    e = launch_scale_f32(g.m, g.n, (float)g.beta, static_cast<float*>(g.C), g.ldc, s);
+    return e == hipSuccess ? JB_OK : hip_fail(e, "scale launch");
+  }
+  if (method == JB_F32_NATIVE) {
+    e = launch_sgemm_mfma(g, s);
+    return e == hipSuccess ? JB_OK : hip_fail(e, "sgemm_mfma launch");
+  }
+  void* ws = nullptr;
+  int rc = ws_acquire(bf16x3_workspace_bytes(g.m, g.n, g.k), s, &ws);
+  if (rc) return rc;
+  e = launch_sgemm_bf16x3(g, nprod, ws, s);
+  rc = ws_release(s);
+  if (e != hipSuccess) return hip_fail(e, "sgemm_bf16x3 launch");
+  return rc;
+}
+
+int jb_sgemm_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
+                 const float* B, int64_t ldb, float beta, float* C, int64_t ldc, void* stream) {
+  JB_GEMM_PROLOGUE(alpha, beta)
+  return sgemm_dev_locked(g, JB_F32_NATIVE, 0, (hipStream_t)stream);
+}
+
+int jb_sgemm(char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
+             const float* B, int64_t ldb, float beta, float* C, int64_t ldc) {
+  JB_GEMM_PROLOGUE(alpha, beta)
+  return run_staged(g, sizeof(float), beta != 0.0f, [](const GemmArgs& d, hipStream_t s) {
+    return sgemm_dev_locked(d, JB_F32_NATIVE, 0, s);
+  });
+}
+
+int jb_sgemm_bf16x3_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha, const float* A,
+                        int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int nprod,
+                        void* stream) {
+  if (nprod != 3 && nprod != 6 && nprod != 9) return fail(-14, "nprod must be 3, 6 or 9 (got %d)", nprod);
+  JB_GEMM_PROLOGUE(alpha, beta)
+  return sgemm_dev_locked(g, JB_F32_BF16X3, nprod, (hipStream_t)stream);
+}
+
+int jb_sgemm_bf16x3(char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha, const float* A,
+                    int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int nprod) {
+  if (nprod != 3 && nprod != 6 && nprod != 9) return fail(-14, "nprod must be 3, 6 or 9 (got %d)", nprod);
+  JB_GEMM_PROLOGUE(alpha, beta)
+  return run_staged(g, sizeof(float), beta != 0.0f, [nprod](const GemmArgs& d, hipStream_t s) {
+    return sgemm_dev_locked(d, JB_F32_BF16X3, nprod, s);
+  });
+}
+
+// ---- generator --------------------------------------------------------------
+int jb_fill_uniform_dev(uint64_t seed, int64_t n, double* out, void* stream) {
+  tl_error.clear();
+  if (n < 0) return fail(-2, "n < 0");
+  if (n > 0 && !out) return fail(-3, "out is NULL");
+  std::lock_guard<std::mutex> lk(rt().mu);
+  if (n == 0) return JB_OK;
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipError_t e = launch_fill_uniform(seed, n, out, (hipStream_t)stream);
+  return e == hipSuccess ? JB_OK : hip_fail(e, "fill launch");
+}
+
+int jb_fill_uniform_f32_dev(uint64_t seed, int64_t n, float* out, void* stream) {
+  tl_error.clear();
+  if (n < 0) return fail(-2, "n < 0");
+  if (n > 0 && !out) return fail(-3, "out is NULL");
+  std::lock_guard<std::mutex> lk(rt().mu);
+  if (n == 0) return JB_OK;
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipError_t e = launch_fill_uniform_f32(seed, n, out, (hipStream_t)stream);
+  return e == hipSuccess ? JB_OK : hip_fail(e, "fill launch");
+}
+
+int jb_fill_wide_dev(uint64_t seed, int64_t n, int emin, int emax, double* out, void* stream) {
+  tl_error.clear();
+  if (n < 0) return fail(-2, "n < 0");
+  if (emin > emax) return fail(-4, "emax < emin");
+  if (emin < -1000 || emax > 1000) return fail(-3, "exponent range outside [-1000, 1000]");
+  if (n > 0 && !out) return fail(-5, "out is NULL");
+  std::lock_guard<std::mutex> lk(rt().mu);
+  if (n == 0) return JB_OK;
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipError_t e = launch_fill_wide(seed, n, emin, emax, out, (hipStream_t)stream);
+  return e == hipSuccess ? JB_OK : hip_fail(e, "fill launch");
+}
+
+}  // extern "C"

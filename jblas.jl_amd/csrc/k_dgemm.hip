@@ -1,0 +1,375 @@
+// k_dgemm.hip -- FP64 GEMM kernels for gfx950.
+//
+// (1) dgemm_mfma: C = alpha*op(A)*op(B) + beta*C on v_mfma_f64_16x16x4f64.
+//     Replaces jmul!'s register-tiled AVX loop (src/gemm.jl:244-348): the CPU
+//     register tile (pick_kernel_size, src/kernel_structure.jl:76-99) becomes a
+//     64x64 wave tile of 4x4 MFMA blocks, and the software prefetch
+//     (src/memory_management.jl:181-277) becomes a double-buffered LDS stage
+//     fed by coalesced 16-byte global loads.  op(A)/op(B) transposition is
+//     absorbed by the loader's LDS write pattern.
+// (2) dgemm_recurrence: one fma chain per element in increasing k, the exact
+//     order of src/gemm.jl:86 (first product) and :165 (fma k-step), for
+//     bit-level parity with the reference recurrence.
+//
+// LDS holds each operand tile k-major ([k][row], rows contiguous) so that an
+// MFMA fragment read is 16 consecutive doubles per 16-lane group.
+#include "jb_internal.h"
+#include <algorithm>
+
+namespace jb {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// 16-byte vector of T.
+template <typename T> struct Vec16;
+template <> struct Vec16<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct Vec16<float> { typedef float type __attribute__((ext_vector_type(4))); };
+
+constexpr int NT = 256;  // threads per block (4 waves)
+constexpr int BK = 16;   // k extent of one LDS stage
+
+// Loads one R x BK operand tile (R rows of op(A) or columns of op(B)) into
+// registers, then writes it k-major into LDS.
+//   MN_CONTIG: element (r, kk) at P[(r0+r) + (k0+kk)*ld]  (A 'N', B 'T')
+//   otherwise: element (r, kk) at P[(k0+kk) + (r0+r)*ld]  (A 'T', B 'N')
+// VEC: 16-byte loads are legal (base 16B aligned, ld a multiple of 16/sizeof(T)).
+template <typename T, int R, bool MN_CONTIG, bool VEC>
+struct TileLoader {
+  static constexpr int VW = 16 / sizeof(T);          // elements per 16-byte vector
+  static constexpr int PER = R * BK / (VW * NT);     // vectors per thread
+  typedef typename Vec16<T>::type V;
+  V v[PER];
+
+  __device__ __forceinline__ void load(const T* __restrict__ P, int64_t ld, int64_t r0, int64_t k0, int64_t Rtot,
+                                       int64_t K, bool full) {
+    const int t = threadIdx.x;
+    if (MN_CONTIG) {
+      constexpr int RV = R / VW;  // vectors per k column
+      const int r = VW * (t % RV), kb = t / RV;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int kk = kb + (NT / RV) * i;
+        const T* src = P + (r0 + r) + (k0 + kk) * ld;
+        if (full && VEC) {
+          v[i] = *reinterpret_cast<const V*>(src);
+        } else if (full) {
+#pragma unroll
+          for (int e = 0; e < VW; ++e) v[i][e] = src[e];
+        } else {
+          const bool kok = (k0 + kk) < K;
+#pragma unroll
+          for (int e = 0; e < VW; ++e) v[i][e] = (kok && r0 + r + e < Rtot) ? src[e] : T(0);
+        }
+      }
+    } else {
+      constexpr int KV = BK / VW;  // vectors per row
+      const int kk = VW * (t % KV), rb = t / KV;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int r = rb + (NT / KV) * i;
+        const T* src = P + (k0 + kk) + (r0 + r) * ld;
+        if (full && VEC) {
+          v[i] = *reinterpret_cast<const V*>(src);
+        } else if (full) {
+#pragma unroll
+          for (int e = 0; e < VW; ++e) v[i][e] = src[e];
+        } else {
+          const bool rok = (r0 + r) < Rtot;
+#pragma unroll
+          for (int e = 0; e < VW; ++e) v[i][e] = (rok && k0 + kk + e < K) ? src[e] : T(0);
+        }
+      }
+    }
+  }
+
+  template <int STRIDE>
+  __device__ __forceinline__ void store(T* S) const {
+    const int t = threadIdx.x;
+    if (MN_CONTIG) {
+      constexpr int RV = R / VW;
+      const int r = VW * (t % RV), kb = t / RV;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) *reinterpret_cast<V*>(S + (kb + (NT / RV) * i) * STRIDE + r) = v[i];
+    } else {
+      constexpr int KV = BK / VW;
+      const int kk = VW * (t % KV), rb = t / KV;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int r = rb + (NT / KV) * i;
+#pragma unroll
+        for (int e = 0; e < VW; ++e) S[(kk + e) * STRIDE + r] = v[i][e];
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// (1) MFMA GEMM: 128x128 block tile, 4 waves of 64x64 (4x4 MFMA blocks of
+//     16x16), BK=16 per LDS stage, two LDS stages.  T = double uses
+//     v_mfma_f64_16x16x4f64, T = float v_mfma_f32_16x16x4f32.
+// ---------------------------------------------------------------------------
+constexpr int MB = 128, NB = 128;
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef d4 acc_t;
+  static __device__ __forceinline__ acc_t mma(double x, double y, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+  }
+  // Row p of the 16x16 result held in item r of a lane of group g = lane/16.
+  static __device__ __forceinline__ int prow(int r, int g) { return 4 * r + g; }
+};
+template <> struct Mfma<float> {
+  typedef f4 acc_t;
+  static __device__ __forceinline__ acc_t mma(float x, float y, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int prow(int r, int g) { return 4 * g + r; }
+};
+
+template <typename T, bool TA, bool TB, bool VEC>
+__global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, int64_t K, T alpha,
+                                                          const T* __restrict__ A, int64_t lda,
+                                                          const T* __restrict__ B, int64_t ldb, T beta,
+                                                          T* __restrict__ C, int64_t ldc, int tiles_m, int tiles_n) {
+  constexpr int STRIDE = MB + 16 / (int)sizeof(T) * 2;  // LDS row stride (elements), 16B-aligned rows
+  __shared__ __attribute__((aligned(16))) T As[2][BK * STRIDE];
+  __shared__ __attribute__((aligned(16))) T Bs[2][BK * STRIDE];
+  typedef typename Mfma<T>::acc_t acc_t;
+
+  const int nblocks = tiles_m * tiles_n;
+  const int tile = xcd_remap(blockIdx.x, nblocks, 8);
+  int tm, tn;
+  tile_coords(tile, tiles_m, tiles_n, 8, tm, tn);
+  const int64_t m0 = (int64_t)tm * MB, n0 = (int64_t)tn * NB;
+
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int wm0 = (wave % 2) * 64, wn0 = (wave / 2) * 64;
+  const int g = lane / 16, l16 = lane % 16;
+
+  TileLoader<T, MB, !TA, VEC> la;
+  TileLoader<T, NB, TB, VEC> lb;
+  const bool fullA = m0 + MB <= M, fullB = n0 + NB <= N;
+
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  const int nkt = (int)((K + BK - 1) / BK);
+  la.load(A, lda, m0, 0, M, K, fullA && BK <= K);
+  lb.load(B, ldb, n0, 0, N, K, fullB && BK <= K);
+  la.template store<STRIDE>(As[0]);
+  lb.template store<STRIDE>(Bs[0]);
+  __syncthreads();
+
+  for (int kt = 0; kt < nkt; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nkt) {
+      const int64_t k0 = (int64_t)(kt + 1) * BK;
+      const bool fk = k0 + BK <= K;
+      la.load(A, lda, m0, k0, M, K, fullA && fk);
+      lb.load(B, ldb, n0, k0, N, K, fullB && fk);
+    }
+    const T* as = As[cur];
+    const T* bs = Bs[cur];
+#pragma unroll
+    for (int t = 0; t < BK / 4; ++t) {
+      // MFMA t covers k = 4t..4t+3; lane group g supplies k = 4t+g.
+      T a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = as[(4 * t + g) * STRIDE + wm0 + 16 * i + l16];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = bs[(4 * t + g) * STRIDE + wn0 + 16 * j + l16];
+      // D'[p][q] = sum_k X[p][k] Y[k][q] with X = op(B)^T block (p -> n) and Y = op(A)^T
+      // block (q -> m), so a 16-lane group holds 16 consecutive rows of one column of C
+      // (coalesced column-major stores).  The f64 MFMA is a sequential fma chain over
+      // its four k values in lane-group order, so each dot product is one fma chain in
+      // increasing k, as in the reference recurrence.
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::mma(b[j], a[i], acc[i][j]);
+    }
+    if (kt + 1 < nkt) {
+      la.template store<STRIDE>(As[cur ^ 1]);
+      lb.template store<STRIDE>(Bs[cur ^ 1]);
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int64_t m = m0 + wm0 + 16 * i + l16;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t n = n0 + wn0 + 16 * j + Mfma<T>::prow(r, g);
+        if (m < M && n < N) {
+          T c = alpha * acc[i][j][r];
+          if (beta != T(0)) c = fma(beta, C[m + n * ldc], c);
+          C[m + n * ldc] = c;
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (2) Sequential-fma FP64 GEMM (bit-exact recurrence): 64x64 block tile,
+//     each thread a 4x4 register tile, BK=16 LDS stage.
+// ---------------------------------------------------------------------------
+constexpr int RB = 64;
+constexpr int RSTRIDE = RB + 2;
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(NT, 2) void dgemm_recurrence_kernel(int64_t M, int64_t N, int64_t K, double alpha,
+                                                                 const double* __restrict__ A, int64_t lda,
+                                                                 const double* __restrict__ B, int64_t ldb,
+                                                                 double beta, double* __restrict__ C,
+                                                                 int64_t ldc, int tiles_m, int tiles_n) {
+  __shared__ __attribute__((aligned(16))) double As[BK * RSTRIDE];
+  __shared__ __attribute__((aligned(16))) double Bs[BK * RSTRIDE];
+  const int nblocks = tiles_m * tiles_n;
+  const int tile = xcd_remap(blockIdx.x, nblocks, 8);
+  int tm, tn;
+  tile_coords(tile, tiles_m, tiles_n, 8, tm, tn);
+  const int64_t m0 = (int64_t)tm * RB, n0 = (int64_t)tn * RB;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+
+  TileLoader<double, RB, !TA, false> la;
+  TileLoader<double, RB, TB, false> lb;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+
+  for (int64_t k0 = 0; k0 < K; k0 += BK) {
+    la.load(A, lda, m0, k0, M, K, false);
+    lb.load(B, ldb, n0, k0, N, K, false);
+    __syncthreads();  // previous stage fully consumed
+    la.template store<RSTRIDE>(As);
+    lb.template store<RSTRIDE>(Bs);
+    __syncthreads();
+    const int kmax = (int)min<int64_t>(BK, K - k0);
+    int kk = 0;
+    if (k0 == 0) {  // acc_1 = fl(a_1 * x_1)   (src/gemm.jl:86)
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[tx + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[ty + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = a[i] * b[j];
+      kk = 1;
+    }
+    for (; kk < kmax; ++kk) {  // acc_n = fma(a_n, x_n, acc_{n-1})   (src/gemm.jl:165)
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk * RSTRIDE + tx + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk * RSTRIDE + ty + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t m = m0 + tx + 16 * i, n = n0 + ty + 16 * j;
+      if (m < M && n < N) {
+        double c = alpha * acc[i][j];
+        if (beta != 0.0) c = fma(beta, C[m + n * ldc], c);
+        C[m + n * ldc] = c;
+      }
+    }
+}
+
+template <typename T>
+__global__ void scale_kernel(int64_t m, int64_t n, T beta, T* __restrict__ C, int64_t ldc) {
+  const int64_t total = m * n;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx % m, j = idx / m;
+    T* p = C + i + j * ldc;
+    *p = (beta == T(0)) ? T(0) : beta * *p;
+  }
+}
+
+bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+}  // namespace
+
+template <typename T>
+static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
+  const int tiles_m = (int)((g.m + MB - 1) / MB), tiles_n = (int)((g.n + NB - 1) / NB);
+  const int nblocks = tiles_m * tiles_n;
+  constexpr int VW = 16 / sizeof(T);
+  const bool vec = aligned16(g.A) && aligned16(g.B) && (g.lda % VW == 0) && (g.ldb % VW == 0);
+  auto A = static_cast<const T*>(g.A);
+  auto B = static_cast<const T*>(g.B);
+  auto C = static_cast<T*>(g.C);
+  const T alpha = (T)g.alpha, beta = (T)g.beta;
+#define JB_LAUNCH(TA, TB, V)                                                                           \
+  hipLaunchKernelGGL((gemm_mfma_kernel<T, TA, TB, V>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, alpha, \
+                     A, g.lda, B, g.ldb, beta, C, g.ldc, tiles_m, tiles_n)
+  if (vec) {
+    if (!g.ta && !g.tb) JB_LAUNCH(false, false, true);
+    else if (!g.ta && g.tb) JB_LAUNCH(false, true, true);
+    else if (g.ta && !g.tb) JB_LAUNCH(true, false, true);
+    else JB_LAUNCH(true, true, true);
+  } else {
+    if (!g.ta && !g.tb) JB_LAUNCH(false, false, false);
+    else if (!g.ta && g.tb) JB_LAUNCH(false, true, false);
+    else if (g.ta && !g.tb) JB_LAUNCH(true, false, false);
+    else JB_LAUNCH(true, true, false);
+  }
+#undef JB_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_dgemm_mfma(const GemmArgs& g, hipStream_t s) { return launch_gemm_mfma<double>(g, s); }
+hipError_t launch_sgemm_mfma(const GemmArgs& g, hipStream_t s) { return launch_gemm_mfma<float>(g, s); }
+
+hipError_t launch_dgemm_recurrence(const GemmArgs& g, hipStream_t s) {
+  const int tiles_m = (int)((g.m + RB - 1) / RB), tiles_n = (int)((g.n + RB - 1) / RB);
+  const int nblocks = tiles_m * tiles_n;
+  auto A = static_cast<const double*>(g.A);
+  auto B = static_cast<const double*>(g.B);
+  auto C = static_cast<double*>(g.C);
+#define JB_LAUNCH(TA, TB)                                                                                 \
+  hipLaunchKernelGGL((dgemm_recurrence_kernel<TA, TB>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, g.alpha, \
+                     A, g.lda, B, g.ldb, g.beta, C, g.ldc, tiles_m, tiles_n)
+  if (!g.ta && !g.tb) JB_LAUNCH(false, false);
+  else if (!g.ta && g.tb) JB_LAUNCH(false, true);
+  else if (g.ta && !g.tb) JB_LAUNCH(true, false);
+  else JB_LAUNCH(true, true);
+#undef JB_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_f64(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipStream_t s) {
+  if (m == 0 || n == 0) return hipSuccess;
+  const int64_t total = m * n;
+  const int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(scale_kernel<double>, dim3(blocks), dim3(256), 0, s, m, n, beta, C, ldc);
+  return hipGetLastError();
+}
+hipError_t launch_scale_f32(int64_t m, int64_t n, float beta, float* C, int64_t ldc, hipStream_t s) {
+  if (m == 0 || n == 0) return hipSuccess;
+  const int64_t total = m * n;
+  const int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(scale_kernel<float>, dim3(blocks), dim3(256), 0, s, m, n, beta, C, ldc);
+  return hipGetLastError();
+}
+
+}  // namespace jb

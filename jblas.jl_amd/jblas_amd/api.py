@@ -1,0 +1,206 @@
+"""Host-side mirror of jBLAS.jl's GEMM entry points over libjblas_hip.so.
+
+Reference interface mirrored (paths relative to the jBLAS.jl repository):
+  jmul!(D, A, X)  "calculates D = A * X"            src/gemm.jl:225-348
+  kernel!(pD, pA, pX, Kernel(...))  D += A*X         src/kernels.jl:239-241
+  initkernel!(pD, pA, pX, Kernel(...))  D = A*X      src/kernels.jl:273-275
+and the BLAS-style gemm! the drop-in exposes (Julia's
+LinearAlgebra.BLAS.gemm!(tA, tB, alpha, A, B, beta, C) argument order).
+
+Operands are column-major matrices, as in Julia:
+  * numpy arrays (host memory) -> the synchronous host entry points;
+  * torch tensors on the GPU whose stride(0) == 1 (a column-major view,
+    e.g. torch.empty(n, m, device="cuda").T) -> the *_dev entry points on the
+    current torch stream.
+Shape mismatches raise DimensionMismatch before any library call (the
+reference rejects them by dispatch: MMatrix{M,P} / {M,N} / {N,P} must agree,
+src/gemm.jl:244).  Library failures raise JBError with the BLAS-style code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import JBError  # noqa: F401  (re-export)
+
+try:  # torch is only needed for device-resident operands
+    import torch
+except Exception:  # pragma: no cover
+    torch = None
+
+
+class DimensionMismatch(ValueError):
+    """Operand shapes do not agree (Julia's DimensionMismatch)."""
+
+
+def _is_torch(x) -> bool:
+    return torch is not None and isinstance(x, torch.Tensor)
+
+
+class _Operand:
+    __slots__ = ("obj", "ptr", "ld", "rows", "cols", "device", "dtype")
+
+    def __init__(self, x, writable: bool, name: str):
+        if _is_torch(x):
+            if x.dim() != 2:
+                raise DimensionMismatch(f"{name} must be 2-D")
+            if not x.is_cuda:
+                raise ValueError(f"{name}: torch tensors must live on the GPU (use numpy for host data)")
+            rows, cols = x.shape
+            if rows > 1 and x.stride(0) != 1:
+                raise ValueError(f"{name}: device matrices must be column-major (stride(0) == 1)")
+            self.obj, self.ptr, self.device = x, x.data_ptr(), True
+            self.ld = max(1, x.stride(1) if cols > 1 else rows)
+            self.dtype = {torch.float64: np.float64, torch.float32: np.float32}.get(x.dtype)
+            if self.dtype is None:
+                raise TypeError(f"{name}: dtype {x.dtype} not supported")
+        else:
+            a = np.asarray(x) if not isinstance(x, np.ndarray) else x
+            if a.ndim != 2:
+                raise DimensionMismatch(f"{name} must be 2-D")
+            if a.dtype not in (np.float64, np.float32):
+                raise TypeError(f"{name}: dtype {a.dtype} not supported")
+            rows, cols = a.shape
+            colmajor = rows <= 1 or a.strides[0] == a.itemsize
+            if not colmajor:
+                if writable:
+                    raise ValueError(f"{name}: output must be column-major (order='F')")
+                a = np.asfortranarray(a)
+            self.obj, self.ptr, self.device = a, a.ctypes.data, False
+            self.ld = max(1, a.strides[1] // a.itemsize if cols > 1 else rows)
+            self.dtype = a.dtype.type
+        self.rows, self.cols = rows, cols
+
+
+def _stream_of(device: bool, stream):
+    if not device:
+        return None
+    if stream is not None:
+        return stream if isinstance(stream, int) else stream.cuda_stream
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _trans(t: str) -> bytes:
+    if t not in ("N", "T", "C", "n", "t", "c"):
+        raise ValueError(f"trans must be 'N', 'T' or 'C', got {t!r}")
+    return t.upper().encode()
+
+
+def _check_shapes(tA, tB, a: _Operand, b: _Operand, c: _Operand):
+    m, k = (a.rows, a.cols) if tA.upper() == "N" else (a.cols, a.rows)
+    k2, n = (b.rows, b.cols) if tB.upper() == "N" else (b.cols, b.rows)
+    if k != k2 or c.rows != m or c.cols != n:
+        raise DimensionMismatch(f"op(A) is {m}x{k}, op(B) is {k2}x{n}, C is {c.rows}x{c.cols}")
+    if not (a.device == b.device == c.device):
+        raise ValueError("operands must all be host (numpy) or all device (torch) arrays")
+    if not (a.dtype == b.dtype == c.dtype):
+        raise TypeError("operands must share one element type")
+    return m, n, k
+
+
+METHODS_F64 = ("native", "recurrence", "ozaki")
+METHODS_F32 = ("native", "bf16x3")
+
+
+def gemm_(tA: str, tB: str, alpha, A, B, beta, C, *, method: str = "native", nslices: int = 7,
+          nprod: int = 6, stream=None):
+    """C = alpha*op(A)*op(B) + beta*C in place; returns C.
+
+    method (Float64): "native" FP64 MFMA, "recurrence" (bit-exact jmul! order),
+    "ozaki" (split-BF16 emulation with `nslices` slices).
+    method (Float32): "native" FP32 MFMA, "bf16x3" (3 BF16 slices, `nprod` products).
+    """
+    a, b, c = _Operand(A, False, "A"), _Operand(B, False, "B"), _Operand(C, True, "C")
+    m, n, k = _check_shapes(tA, tB, a, b, c)
+    ta, tb = _trans(tA), _trans(tB)
+    dev = a.device
+    s = _stream_of(dev, stream)
+    common = (ta, tb, m, n, k)
+    if c.dtype == np.float64:
+        if method == "native":
+            name = "jb_dgemm_dev" if dev else "jb_dgemm"
+            args = common + (float(alpha), a.ptr, a.ld, b.ptr, b.ld, float(beta), c.ptr, c.ld)
+        elif method == "recurrence":
+            name = "jb_dgemm_recurrence_dev" if dev else "jb_dgemm_recurrence"
+            args = common + (float(alpha), a.ptr, a.ld, b.ptr, b.ld, float(beta), c.ptr, c.ld)
+        elif method == "ozaki":
+            name = "jb_dgemm_ozaki_dev" if dev else "jb_dgemm_ozaki"
+            args = common + (float(alpha), a.ptr, a.ld, b.ptr, b.ld, float(beta), c.ptr, c.ld, int(nslices))
+        else:
+            raise ValueError(f"Float64 method must be one of {METHODS_F64}")
+    else:
+        if method == "native":
+            name = "jb_sgemm_dev" if dev else "jb_sgemm"
+            args = common + (float(alpha), a.ptr, a.ld, b.ptr, b.ld, float(beta), c.ptr, c.ld)
+        elif method == "bf16x3":
+            name = "jb_sgemm_bf16x3_dev" if dev else "jb_sgemm_bf16x3"
+            args = common + (float(alpha), a.ptr, a.ld, b.ptr, b.ld, float(beta), c.ptr, c.ld, int(nprod))
+        else:
+            raise ValueError(f"Float32 method must be one of {METHODS_F32}")
+    if dev:
+        args = args + (s,)
+    _lib.call(name, *args)
+    return C
+
+
+def jmul_(D, A, X, *, stream=None):
+    """D = A*X (jmul!, src/gemm.jl:244-348): the reference's per-element fma
+    recurrence, bit-identical, with the fringe rows/columns written too.
+    Float64 only, like the reference's tested path."""
+    d, a, x = _Operand(D, True, "D"), _Operand(A, False, "A"), _Operand(X, False, "X")
+    M, N = a.rows, a.cols
+    if x.rows != N or d.rows != M or d.cols != x.cols:
+        raise DimensionMismatch(f"D is {d.rows}x{d.cols}, A is {M}x{N}, X is {x.rows}x{x.cols}")
+    if not (d.dtype == a.dtype == x.dtype == np.float64):
+        raise TypeError("jmul! drop-in takes Float64 matrices")
+    P = x.cols
+    if d.ld != max(1, M) or a.ld != max(1, M) or x.ld != max(1, N):
+        # Strided views go through the BLAS form with explicit leading dimensions.
+        return gemm_("N", "N", 1.0, A, X, 0.0, D, method="recurrence", stream=stream)
+    if d.device:
+        _lib.call("jb_jmul_dev", M, N, P, d.ptr, a.ptr, x.ptr, _stream_of(True, stream))
+    else:
+        _lib.call("jb_jmul", M, N, P, d.ptr, a.ptr, x.ptr)
+    return D
+
+
+def kernel_(D, A, X, *, stream=None):
+    """D += A*X on (sub)matrices with their own leading dimensions
+    (kernel!, src/kernels.jl:212-241): gemm! with alpha = beta = 1."""
+    return gemm_("N", "N", 1.0, A, X, 1.0, D, method="recurrence", stream=stream)
+
+
+def initkernel_(D, A, X, *, stream=None):
+    """D = A*X on (sub)matrices (initkernel!, src/kernels.jl:242-275)."""
+    return gemm_("N", "N", 1.0, A, X, 0.0, D, method="recurrence", stream=stream)
+
+
+# ---- seeded device generator (mrandn replacement, src/randmat.jl:11-14) ------
+def fill_uniform_(out, seed: int, *, stream=None):
+    """Fill a contiguous device tensor with SplitMix64 uniforms in [-1, 1)."""
+    if not (_is_torch(out) and out.is_cuda and out.is_contiguous()):
+        raise ValueError("out must be a contiguous GPU tensor")
+    s = _stream_of(True, stream)
+    if out.dtype == torch.float64:
+        _lib.call("jb_fill_uniform_dev", seed, out.numel(), out.data_ptr(), s)
+    elif out.dtype == torch.float32:
+        _lib.call("jb_fill_uniform_f32_dev", seed, out.numel(), out.data_ptr(), s)
+    else:
+        raise TypeError("float64 or float32 only")
+    return out
+
+
+def fill_wide_(out, seed: int, emin: int = -12, emax: int = 12, *, stream=None):
+    """Fill with u*2^e, u uniform in [-1,1), e uniform integer in [emin, emax]."""
+    if not (_is_torch(out) and out.is_cuda and out.is_contiguous() and out.dtype == torch.float64):
+        raise ValueError("out must be a contiguous float64 GPU tensor")
+    _lib.call("jb_fill_wide_dev", seed, out.numel(), emin, emax, out.data_ptr(), _stream_of(True, stream))
+    return out
+
+
+def device_matrix(rows: int, cols: int, dtype=None, device="cuda"):
+    """An uninitialised column-major device matrix (stride(0) == 1)."""
+    dtype = dtype or torch.float64
+    return torch.empty((cols, rows), dtype=dtype, device=device).T

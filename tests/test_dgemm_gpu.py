@@ -1,0 +1,229 @@
+"""GPU parity of the native FP64/FP32 MFMA GEMM and the bit-exact recurrence
+kernel, through the C ABI, against the oracle (oracle/oracle.c).
+
+Bars: the jmul! recurrence and the FP64 MFMA path are compared BIT-FOR-BIT
+with the oracle's restatement of the reference recurrence (the f64 MFMA is a
+sequential fma chain in k order); against the Dot2 FP64 oracle the stated
+tolerance is the componentwise bound
+    |C - C*|_ij <= 2*K*u * (|alpha| |op(A)||op(B)| + |beta| |C0|)_ij,  u = 2^-53.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import jblas_amd as jb
+from jblas_amd import generators as G
+from oracle import oracle as O
+
+from conftest import to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+U = 2.0 ** -53
+
+
+def opnd(x, t):
+    return x if t == "N" else np.asfortranarray(x.T)
+
+
+def scaled_err(got, ref, absab):
+    return float(np.max(np.abs(got - ref) / np.where(absab > 0, absab, 1.0)))
+
+
+# ---------------------------------------------------------------------------
+def test_device_properties(gpu):
+    info = jb.device_info()
+    print(info)
+    assert info["compute_units"] > 0 and info["wavefront_size"] == 64
+    assert "gfx950" in info["arch"]
+    assert abs(info["peak_fp64_tflops"] - info["compute_units"] * info["clock_khz"] * 1e-9 * 128) < 1e-6
+
+
+@pytest.mark.parametrize("m,n,k", [(16, 16, 4), (64, 64, 16), (128, 128, 16), (130, 70, 33), (257, 300, 129)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_mfma_layout_integer_kat(gpu, m, n, k, dtype):
+    # small integers: every summation order is exact, so any lane-layout or index error shows
+    rng = np.random.default_rng(m * 7 + n + k)
+    A = np.asfortranarray(rng.integers(-8, 9, (m, k)).astype(dtype))
+    B = np.asfortranarray(rng.integers(-8, 9, (k, n)).astype(dtype))
+    ref = (A.astype(np.int64) @ B.astype(np.int64)).astype(dtype)
+    C = np.zeros((m, n), dtype=dtype, order="F")
+    jb.gemm_("N", "N", 1.0, A, B, 0.0, C)
+    assert np.array_equal(C, ref)
+    for ta in "NT":
+        for tb in "NT":
+            Cd = to_dev(gpu, np.zeros((m, n), dtype=dtype))
+            jb.gemm_(ta, tb, 1.0, to_dev(gpu, opnd(A, ta)), to_dev(gpu, opnd(B, tb)), 0.0, Cd)
+            gpu.cuda.synchronize()
+            assert np.array_equal(to_host(Cd), ref), (ta, tb)
+
+
+def test_jmul_c1_bitexact(gpu):
+    """Config: jmul! C = A*B, Float64, M=N=K=256, uniform[-1,1) seeds 42/43, C zero."""
+    A = G.matrix(42, 256, 256)
+    B = G.matrix(43, 256, 256)
+    ref = O.jmul(A, B)
+    D = np.zeros((256, 256), order="F")
+    jb.jmul_(D, A, B)
+    assert np.array_equal(D, ref)
+    D2 = np.zeros((256, 256), order="F")
+    jb.gemm_("N", "N", 1.0, A, B, 0.0, D2)  # FP64 MFMA path: same fma chain
+    assert np.array_equal(D2, ref)
+    Dd = to_dev(gpu, np.zeros((256, 256)))
+    jb.jmul_(Dd, to_dev(gpu, A), to_dev(gpu, B))
+    gpu.cuda.synchronize()
+    assert np.array_equal(to_host(Dd), ref)
+    # within the stated tolerance of the Dot2 oracle as well
+    d2 = O.gemm_dot2("N", "N", 1.0, A, B, 0.0, np.zeros((256, 256)))
+    assert scaled_err(D, d2, np.abs(A) @ np.abs(B)) <= 2 * 256 * U
+
+
+@pytest.mark.parametrize("M,N,P", [(16, 32, 14), (32, 32, 28), (128, 128, 126), (32, 32, 6), (32, 32, 24),
+                                   (13, 12, 12), (24, 18, 18), (800, 900, 840), (1, 1, 1), (3, 1000, 5)])
+def test_jmul_reference_shapes(gpu, M, N, P):
+    """The shapes test/runtests.jl exercises (plus fringe shapes the reference skips)."""
+    A = G.matrix(1, M, N)
+    X = G.matrix(2, N, P)
+    D = np.full((M, P), np.nan, order="F")
+    jb.jmul_(D, A, X)
+    assert np.array_equal(D, O.jmul(A, X))
+
+
+def test_jmul_sign_of_zero(gpu):
+    D = np.ones((1, 1), order="F")
+    jb.jmul_(D, np.array([[-1.0]]), np.array([[0.0]]))
+    assert D[0, 0] == 0.0 and math.copysign(1.0, D[0, 0]) < 0  # acc_1 = fl(a*x), src/gemm.jl:86
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_dgemm_c2_1024(gpu, ta, tb):
+    """Config: C = 1.5*op(A)*op(B) + 0.5*C, M=N=K=1024, uniform[-1,1) seeds 42/43/44."""
+    n = 1024
+    A, B, C0 = G.matrix(42, n, n), G.matrix(43, n, n), G.matrix(44, n, n)
+    Ad, Bd = to_dev(gpu, A), to_dev(gpu, B)
+    ref = O.gemm_recurrence(ta, tb, 1.5, A, B, 0.5, C0)
+    for method in ("native", "recurrence"):
+        Cd = to_dev(gpu, C0)
+        jb.gemm_(ta, tb, 1.5, Ad, Bd, 0.5, Cd, method=method)
+        gpu.cuda.synchronize()
+        got = to_host(Cd)
+        assert np.array_equal(got, ref), method
+    ii = np.random.default_rng(1).integers(0, n, 4096)
+    jj = np.random.default_rng(2).integers(0, n, 4096)
+    d2 = O.gemm_dot2_sample(ta, tb, 1.5, A, B, 0.5, C0[ii, jj], ii, jj)
+    absab = 1.5 * (np.abs(opnd(A, ta)) @ np.abs(opnd(B, tb))) + 0.5 * np.abs(C0)
+    assert scaled_err(got[ii, jj], d2, absab[ii, jj]) <= 2 * n * U
+
+
+@pytest.mark.parametrize("n,ta,tb", [(4096, "N", "N"), (4096, "T", "N"), (8192, "N", "N"), (8192, "N", "T")])
+def test_dgemm_c2_large_sampled(gpu, n, ta, tb):
+    """Config sizes 4096/8192: inputs generated in HBM (bit-identical to the host
+    generator), sampled entries bit-checked against the recurrence and within
+    tolerance of Dot2."""
+    torch = gpu
+    Ad = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    Bd = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    Cd = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    jb.fill_uniform_(Ad, 42)
+    jb.fill_uniform_(Bd, 43)
+    jb.fill_uniform_(Cd, 44)
+    A, B, C0 = to_host(Ad.T), to_host(Bd.T), to_host(Cd.T)  # column-major views of the streams
+    assert np.array_equal(A[:, 0], G.uniform(42, n))
+    jb.gemm_(ta, tb, 1.5, Ad.T, Bd.T, 0.5, Cd.T)
+    torch.cuda.synchronize()
+    got = to_host(Cd.T)
+    rng = np.random.default_rng(n)
+    ii = np.concatenate([rng.integers(0, n, 2048), np.full(64, n - 1), np.arange(64)])
+    jj = np.concatenate([rng.integers(0, n, 2048), np.arange(64), np.full(64, n - 1)])
+    rec = O.gemm_recurrence_sample(ta, tb, 1.5, A, B, 0.5, C0[ii, jj], ii, jj)
+    assert np.array_equal(got[ii, jj], rec)
+    d2 = O.gemm_dot2_sample(ta, tb, 1.5, A, B, 0.5, C0[ii, jj], ii, jj)
+    Ao, Bo = opnd(A, ta), opnd(B, tb)
+    absab = 1.5 * np.einsum("sk,ks->s", np.abs(Ao[ii, :]), np.abs(Bo[:, jj])) + 0.5 * np.abs(C0[ii, jj])
+    assert scaled_err(got[ii, jj], d2, absab) <= 2 * n * U
+    # size-independent property: linearity in B, C(A, 2B) == 2*C(A, B) exactly (power of two)
+    C2 = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    C3 = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    jb.gemm_(ta, tb, 1.0, Ad.T, Bd.T, 0.0, C2.T)
+    jb.gemm_(ta, tb, 1.0, Ad.T, (2 * Bd).T, 0.0, C3.T)
+    torch.cuda.synchronize()
+    assert torch.equal(C3, 2 * C2)
+    del Ad, Bd, Cd, C2, C3
+    torch.cuda.empty_cache()
+
+
+def test_strided_subviews(gpu):
+    """kernel!/initkernel! semantics: sub-blocks with leading dimensions != rows
+    (src/kernels.jl:212-275), host and device."""
+    big = G.matrix(7, 300, 300)
+    A = big[10:110, 5:85]       # 100 x 80, lda 300
+    B = big[20:100, 100:170]    # 80 x 70
+    Cbig = G.matrix(8, 301, 90)
+    C = Cbig[1:101, 3:73]       # 100 x 70, ldc 301
+    ref = O.gemm_recurrence("N", "N", 1.0, A, B, 1.0, C)
+    Cw = np.asfortranarray(Cbig.copy())
+    jb.kernel_(Cw[1:101, 3:73], A, B)  # D += A*X
+    assert np.array_equal(Cw[1:101, 3:73], ref)
+    assert np.array_equal(Cw[0, :], Cbig[0, :]) and np.array_equal(Cw[:, 80:], Cbig[:, 80:])
+    bigd = to_dev(gpu, big)
+    Cd = to_dev(gpu, Cbig)
+    jb.gemm_("N", "N", 1.0, bigd[10:110, 5:85], bigd[20:100, 100:170], 1.0, Cd[1:101, 3:73])
+    gpu.cuda.synchronize()
+    out = to_host(Cd)
+    assert np.array_equal(out[1:101, 3:73], ref)
+    assert np.array_equal(out[101:, :], Cbig[101:, :])
+
+
+def test_beta_zero_never_reads_c_alpha_zero_k_zero(gpu):
+    A, B = G.matrix(3, 70, 50), G.matrix(4, 50, 60)
+    C = np.full((70, 60), np.nan, order="F")
+    jb.gemm_("N", "N", 1.0, A, B, 0.0, C)
+    assert not np.isnan(C).any()
+    C0 = G.matrix(5, 70, 60)
+    C = C0.copy(order="F")
+    Anan = np.full((70, 50), np.nan, order="F")
+    jb.gemm_("N", "N", 0.0, Anan, B, 0.5, C)  # alpha == 0: A not referenced
+    assert np.array_equal(C, 0.5 * C0)
+    C = C0.copy(order="F")
+    jb.gemm_("N", "N", 2.0, np.zeros((70, 0), order="F"), np.zeros((0, 60), order="F"), -1.0, C)
+    assert np.array_equal(C, -C0)
+
+
+def test_fill_bit_identical(gpu):
+    torch = gpu
+    t = torch.empty(100003, dtype=torch.float64, device="cuda")
+    jb.fill_uniform_(t, 123)
+    assert np.array_equal(t.cpu().numpy(), G.uniform(123, 100003))
+    jb.fill_wide_(t, 9, -12, 12)
+    assert np.array_equal(t.cpu().numpy(), G.wide(9, 100003))
+    f = torch.empty(1001, dtype=torch.float32, device="cuda")
+    jb.fill_uniform_(f, 5)
+    assert np.array_equal(f.cpu().numpy(), G.uniform(5, 1001).astype(np.float32))
+
+
+def test_sgemm_native_tolerance(gpu):
+    n = 512
+    A = G.matrix(42, n, n, dtype=np.float32)
+    B = G.matrix(43, n, n, dtype=np.float32)
+    C0 = G.matrix(44, n, n, dtype=np.float32)
+    ref = O.sgemm_dot2("N", "T", 1.5, A, opnd(B, "T"), 0.5, C0)
+    C = C0.copy(order="F")
+    jb.gemm_("N", "T", 1.5, A, opnd(B, "T"), 0.5, C)
+    absab = 1.5 * (np.abs(A).astype(float) @ np.abs(B).astype(float)) + 0.5 * np.abs(C0)
+    assert scaled_err(C.astype(float), ref, absab) <= 2 * n * 2.0 ** -24
+
+
+def test_alloc_failure_is_reported(gpu):
+    n = 2048  # larger host staging than any earlier host-pointer call in this session
+    A, B = G.matrix(1, n, n), G.matrix(2, n, n)
+    C = np.zeros((n, n), order="F")
+    os.environ["JB_FAULT_ALLOC"] = "1"
+    try:
+        with pytest.raises(jb.JBError) as ei:
+            jb.gemm_("N", "N", 1.0, A, B, 0.0, C)
+        assert ei.value.code == 2
+    finally:
+        del os.environ["JB_FAULT_ALLOC"]
+    jb.gemm_("N", "N", 1.0, A, B, 0.0, C)
+    assert np.isfinite(C).all()
