@@ -327,14 +327,23 @@ ORC_API void orc_sgemm_dot2(char ta, char tb, int64_t m, int64_t n, int64_t k, d
  *   x = ldexp(a, -E_i) in (-1, 1);  r = x*2^7;  d_1 = rint(r); r -= d_1;
  *   for t >= 2: r *= 2^8; d_t = rint(r); r -= d_t.      (|d_t| <= 128)
  *   so  a ~= 2^E_i * sum_t d_t * 2^(-7-8(t-1)).
- *   Level sums  S_l = sum_{t+u=l, t,u<=s} sum_p dA_t[i,p]*dB_u[p,j]  (exact),
- *   for l = 2..s+1 (triangular truncation).
- *   Groups g = 0,1,2 hold levels 2+3g..4+3g:  G_g = sum 2^(-8(l-l_g)) S_l  (exact).
- *   v = compensated sum over g from the smallest: w_g = ldexp(G_g, -24g);
+ *   Level sums over one k-chunk [c0, c0+kc):
+ *     S_l = sum_{t+u=l, t,u<=s} sum_{p in chunk} dA_t[i,p]*dB_u[p,j]   (exact integers),
+ *   for l = 2..s+1 (triangular truncation).  kc = largest power of two with
+ *   kc * s <= 1024, so that every level sum of a chunk stays below 2^24 and is
+ *   exact in FP32 whatever the summation order.
+ *   After each chunk, in increasing l:  g = (l-2)/4, j = (l-2)%4,
+ *     G_g = fma((double)S_l, 2^(-8j), G_g)            (groups of four levels).
+ *   v = compensated sum over g from the smallest: w_g = ldexp(G_g, -32g);
  *       (hi, e) = TwoSum(hi, w_g), lo += e;  v = hi + lo;
  *   acc = ldexp(v, E_i + F_j - 14);  c = alpha*acc;  c = fma(beta, c_old, c) if beta != 0.
  * Rows/columns containing a non-finite value make the routine return 1.
  */
+ORC_API int orc_ozaki_chunk(int s) {
+  int kc = 1024;
+  while (kc * s > 1024) kc >>= 1;
+  return kc;
+}
 static int row_exponent(const double* v, int64_t n, int64_t stride, int* e) {
   double mx = 0.0;
   for (int64_t p = 0; p < n; ++p) {
@@ -387,22 +396,28 @@ ORC_API int orc_ozaki_split_b(char tb, int64_t k, int64_t n, const double* B, in
 
 static double ozaki_entry(int64_t m, int64_t n, int64_t k, int s, const int* eA, const int16_t* dA,
                           const int* eB, const int16_t* dB, int64_t i, int64_t j) {
-  int64_t S[16] = {0};
-  for (int t = 1; t <= s; ++t)
-    for (int u = 1; u + t <= s + 1; ++u) {
-      const int16_t* a = dA + (int64_t)(t - 1) * m * k + i * k;
-      const int16_t* b = dB + (int64_t)(u - 1) * n * k + j * k;
-      int64_t acc = 0;
-      for (int64_t p = 0; p < k; ++p) acc += (int64_t)a[p] * (int64_t)b[p];
-      S[t + u] += acc;
+  const int kc = orc_ozaki_chunk(s);
+  const int ngroups = (s + 3) / 4; /* levels 2..s+1 -> groups of 4 */
+  double G[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t c0 = 0; c0 < k; c0 += kc) {
+    const int64_t c1 = c0 + kc < k ? c0 + kc : k;
+    int64_t S[20] = {0};
+    for (int t = 1; t <= s; ++t)
+      for (int u = 1; u + t <= s + 1; ++u) {
+        const int16_t* a = dA + (int64_t)(t - 1) * m * k + i * k;
+        const int16_t* b = dB + (int64_t)(u - 1) * n * k + j * k;
+        int64_t acc = 0;
+        for (int64_t p = c0; p < c1; ++p) acc += (int64_t)a[p] * (int64_t)b[p];
+        S[t + u] += acc;
+      }
+    for (int l = 2; l <= s + 1; ++l) {
+      const int g = (l - 2) / 4, jj = (l - 2) % 4;
+      G[g] = fma((double)S[l], ldexp(1.0, -8 * jj), G[g]);
     }
-  const int ngroups = (s + 2) / 3; /* levels 2..s+1 -> groups of 3 */
+  }
   double hi = 0.0, lo = 0.0;
   for (int g = ngroups - 1; g >= 0; --g) {
-    const int l0 = 2 + 3 * g;
-    double G = 0.0;
-    for (int l = l0; l <= l0 + 2 && l <= s + 1; ++l) G += ldexp((double)S[l], -8 * (l - l0));
-    double w = ldexp(G, -24 * g), e;
+    double w = ldexp(G[g], -32 * g), e;
     two_sum(hi, w, &hi, &e);
     lo += e;
   }

@@ -61,6 +61,8 @@ def lib():
     L.orc_ozaki_split_a.argtypes = [C.c_char, i64, i64, p, i64, C.c_int, p, p]
     L.orc_ozaki_split_b.argtypes = [C.c_char, i64, i64, p, i64, C.c_int, p, p]
     L.orc_dgemm_ozaki.argtypes = [C.c_char, C.c_char, i64, i64, i64, dbl, p, i64, p, i64, dbl, p, i64, C.c_int, C.c_int]
+    L.orc_ozaki_chunk.argtypes = [C.c_int]
+    L.orc_ozaki_chunk.restype = C.c_int
     L.orc_pick_kernel_size.argtypes = [i64, i64, i64, p]
     L.orc_blocking_structure.argtypes = [i64, i64, i64, i64, p, i64, i64, p]
     for fn in ("orc_jmul_d", "orc_jmul_s", "orc_dgemm_ozaki", "orc_ozaki_split_a", "orc_ozaki_split_b",
@@ -224,6 +226,11 @@ def dgemm_ozaki(ta, tb, alpha, A, B, beta, C, s, nthreads=None) -> np.ndarray:
     if rc:
         raise ValueError(f"orc_dgemm_ozaki rc={rc}")
     return C
+
+
+def ozaki_chunk(s: int) -> int:
+    """k-chunk length over which the emulated path's FP32 level sums stay exact."""
+    return int(lib().orc_ozaki_chunk(s))
 
 
 def pick_kernel_size(register_size: int, register_count: int, t_size: int):
