@@ -1,0 +1,280 @@
+"""Benchmark of the GEMM hot path on one MI355X.
+
+Metric: GEMM TFLOP/s at 8192^3, split-BF16 emulated DGEMM vs native FP64
+DGEMM, with the max relative error of each against the FP64 (Dot2) oracle.
+
+A step is one C = A*B at M = N = K = 8192 (column-major, NN) on inputs already
+resident in HBM, a_ij = u*2^e with u uniform in [-1,1) and e uniform in
+[-12, 12] (the emulated-DGEMM workload), computed by the split-BF16 emulated
+DGEMM with `--nslices` slices (default 7: the smallest count whose error is
+within 4x native FP64's on this workload).  `value` is that step's
+throughput, 2*M*N*K / step time.  The native FP64-MFMA DGEMM is timed on the
+same inputs for the comparison, and a slice-count table (3..8) is measured
+after the timed region.
+
+Only one GPU: `--gpus N` with N > 1 exits with a message.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "jblas.jl_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "GEMM TFLOP/s at 8192³: split-BF16 emulated vs native FP64 DGEMM, max rel err"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--nslices", type=int, default=7)
+    ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline and the oracle check")
+    ap.add_argument("--skip-table", action="store_true", help="omit the per-slice-count table")
+    ap.add_argument("--only", choices=["ozaki", "native"], default=None,
+                    help="run only one method (for profiler runs)")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed(torch, fn, steps):
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def cpu_baseline_leg(A_sub_rows, B_sub_cols, A_panel, B_panel, nthreads):
+    """CPU side, rank 0 only: (1) the oracle's restatement of the reference
+    jmul! recurrence timed on a bounded panel of the same workload, (2) the
+    Dot2 FP64 oracle on the sampled rows x columns, used only as the checker."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    out = {}
+    # (1) reference CPU path (jmul! restatement), all host threads and one thread
+    t0 = time.perf_counter()
+    O.jmul(A_panel, B_panel, nthreads=nthreads)
+    t_mt = time.perf_counter() - t0
+    flops = 2.0 * A_panel.shape[0] * A_panel.shape[1] * B_panel.shape[1]
+    Bp1 = np.asfortranarray(B_panel[:, : int(os.environ.get("JB_CPU_PANEL_1T", "160"))])
+    t0 = time.perf_counter()
+    O.jmul(A_panel, Bp1, nthreads=1)
+    t_1 = time.perf_counter() - t0
+    flops1 = 2.0 * A_panel.shape[0] * A_panel.shape[1] * Bp1.shape[1]
+    out["cpu"] = {"mt_tflops": flops / t_mt / 1e12, "mt_seconds": t_mt, "mt_threads": nthreads,
+                  "st_tflops": flops1 / t_1 / 1e12, "st_seconds": t_1,
+                  "panel": f"D[:, 0:{B_panel.shape[1]}] = A*B[:, 0:{B_panel.shape[1]}] of the "
+                           f"{A_panel.shape[0]}x{A_panel.shape[1]}x{A_panel.shape[1]} workload "
+                           f"(single-thread leg: {Bp1.shape[1]} columns)"}
+    # (2) checker: Dot2 on the sampled block
+    out["ref"] = O.gemm_dot2("N", "N", 1.0, A_sub_rows, B_sub_cols, 0.0,
+                             np.zeros((A_sub_rows.shape[0], B_sub_cols.shape[1])), nthreads=nthreads)
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
+                                   "DESIGN.md and not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import jblas_amd as jb
+    from jblas_amd import _lib
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    jb.init(0)
+    info = jb.device_info()
+    n, s = args.n, args.nslices
+    flops = 2.0 * n * n * n
+    pairs = s * (s + 1) // 2
+    log(f"device {info['arch']} CUs={info['compute_units']} clock={info['clock_khz'] / 1e6:.2f}GHz "
+        f"peak fp64={info['peak_fp64_tflops']:.1f} bf16={info['peak_bf16_tflops']:.1f} TF")
+
+    # inputs resident in HBM (seeded device generator, bit-identical to the host one)
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    B = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    C = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    jb.fill_wide_(A, 42, -12, 12)
+    jb.fill_wide_(B, 43, -12, 12)
+    Ac, Bc, Cc = A.T, B.T, C.T  # column-major views
+
+    def step_ozaki():
+        jb.gemm_("N", "N", 1.0, Ac, Bc, 0.0, Cc, method="ozaki", nslices=s)
+
+    def step_native():
+        jb.gemm_("N", "N", 1.0, Ac, Bc, 0.0, Cc, method="native")
+
+    result = {}
+    # ---- emulated DGEMM (the measured step) ----
+    if args.only in (None, "ozaki"):
+        for _ in range(args.warmup):
+            step_ozaki()
+        torch.cuda.synchronize()
+        _lib.kernel_timing_reset()
+        _lib.kernel_timing(True)
+        ms = timed(torch, step_ozaki, args.steps)
+        _lib.kernel_timing(False)
+        k_ms, k_n = _lib.kernel_time("slice_pair_gemm")
+        sp_ms, sp_n = _lib.kernel_time("ozaki_split")
+        result["ozaki"] = {"ms": ms, "tflops": flops / ms / 1e9, "gemm_kernel_ms": k_ms / max(k_n, 1),
+                           "split_ms": sp_ms / max(sp_n, 1), "launches": k_n}
+        log(f"ozaki s={s}: {ms:.3f} ms/step  {flops / ms / 1e9:.2f} TF  (slice-pair kernel "
+            f"{k_ms / max(k_n, 1):.3f} ms, split {sp_ms / max(sp_n, 1):.3f} ms)")
+        C_oz = C.clone() if not args.skip_cpu else None
+    # ---- native FP64 DGEMM (comparison) ----
+    if args.only in (None, "native"):
+        for _ in range(args.warmup):
+            step_native()
+        _lib.kernel_timing_reset()
+        _lib.kernel_timing(True)
+        ms_n = timed(torch, step_native, args.steps)
+        _lib.kernel_timing(False)
+        kn_ms, kn_n = _lib.kernel_time("dgemm_mfma")
+        result["native"] = {"ms": ms_n, "tflops": flops / ms_n / 1e9, "kernel_ms": kn_ms / max(kn_n, 1)}
+        log(f"native: {ms_n:.3f} ms/step  {flops / ms_n / 1e9:.2f} TF")
+        C_nat = C.clone() if not args.skip_cpu else None
+
+    # ---- slice-count table (after the timed region) ----
+    table = []
+    if args.only is None and not args.skip_table:
+        for ss in range(3, 9):
+            def f(ss=ss):
+                jb.gemm_("N", "N", 1.0, Ac, Bc, 0.0, Cc, method="ozaki", nslices=ss)
+            f()
+            ms_s = timed(torch, f, 2)
+            table.append({"nslices": ss, "ms": round(ms_s, 3), "tflops": round(flops / ms_s / 1e9, 2),
+                          "out": C.clone() if not args.skip_cpu else None})
+            log(f"  table s={ss}: {ms_s:.3f} ms  {flops / ms_s / 1e9:.2f} TF")
+
+    # ---- CPU baseline + checker (rank 0, N = 1) ----
+    cpu = None
+    errs = {}
+    if not args.skip_cpu:
+        rng = np.random.default_rng(88)
+        ii = np.sort(rng.choice(n, 64, replace=False))
+        jj = np.sort(rng.choice(n, 64, replace=False))
+        Ah = A.T.contiguous().cpu().numpy().T  # column-major host copies
+        Bh = B.T.contiguous().cpu().numpy().T
+        A_rows = np.asfortranarray(Ah[ii, :])
+        B_cols = np.asfortranarray(Bh[:, jj])
+        panel_cols = int(os.environ.get("JB_CPU_PANEL", "2560"))
+        nthreads = max(1, min(16, os.cpu_count() or 1))
+        leg = cpu_baseline_leg(A_rows, B_cols, np.asfortranarray(Ah), np.asfortranarray(Bh[:, :panel_cols]), nthreads)
+        del Ah, Bh
+        ref = leg["ref"]
+        absab = np.abs(A_rows) @ np.abs(B_cols)
+
+        def err(out_t):
+            got = out_t.T.contiguous().cpu().numpy().T[np.ix_(ii, jj)]
+            return (float(np.max(np.abs(got - ref) / np.abs(ref))), float(np.max(np.abs(got - ref) / absab)))
+
+        if "ozaki" in result:
+            errs["ozaki"] = err(C_oz)
+        if "native" in result:
+            errs["native"] = err(C_nat)
+        for row in table:
+            row["max_rel_err"], row["max_err_over_absAB"] = err(row.pop("out"))
+            if "native" in errs:
+                row["err_ratio_vs_native"] = round(row["max_rel_err"] / errs["native"][0], 3)
+        cpu = leg["cpu"]
+        log(f"errors: {errs}; cpu: {cpu}")
+    else:
+        for row in table:
+            row.pop("out", None)
+
+    oz = result.get("ozaki")
+    nat = result.get("native")
+    main_ms = oz["ms"] if oz else nat["ms"]
+    line = {
+        "metric": METRIC,
+        "value": round(flops / main_ms / 1e9, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if oz else "f64",
+        "data": "synthetic (seeded SplitMix64 in HBM: a_ij = u*2^e, u in [-1,1), e in [-12,12])",
+        "config": {"workload": f"dgemm {n}x{n}x{n} column-major NN, split-BF16 emulated with {s} slices "
+                               f"({pairs} BF16 slice-pair products, FP32 exact level sums, FP64 combine) "
+                               f"vs native FP64 MFMA",
+                   "m": n, "n": n, "k": n, "nslices": s, "parallelism": "single GPU"},
+    }
+    if oz:
+        line["emulated_tflops"] = round(oz["tflops"], 3)
+    if nat:
+        line["native_fp64_tflops"] = round(nat["tflops"], 3)
+        line["native_fp64_frac_of_peak"] = round(nat["tflops"] / info["peak_fp64_tflops"], 4)
+    if oz and nat:
+        line["speedup_emulated_vs_native"] = round(nat["ms"] / oz["ms"], 4)
+    if errs:
+        if "ozaki" in errs:
+            line["max_rel_err_emulated"] = errs["ozaki"][0]
+        if "native" in errs:
+            line["max_rel_err_native"] = errs["native"][0]
+        if "ozaki" in errs and "native" in errs:
+            line["err_ratio_emulated_vs_native"] = round(errs["ozaki"][0] / errs["native"][0], 4)
+        line["error_sample"] = "Dot2 oracle on 64 sampled rows x 64 sampled columns (4096 entries)"
+    # roofline of the dominant kernel
+    if oz:
+        hw_flops = pairs * flops
+        ach = hw_flops / (oz["gemm_kernel_ms"] * 1e-3) / 1e12
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tpath):
+            try:
+                with open(tpath) as f:
+                    traffic = json.load(f).get(f"slice_pair_gemm_s{s}_n{n}")
+            except Exception:
+                traffic = None
+        line["roofline"] = {"kernel": f"slice_pair_gemm_kernel<0,{s},{s + 1},1>", "bound": "mfma",
+                            "achieved": round(ach, 2), "peak": round(info["peak_bf16_tflops"], 1),
+                            "unit": "TFLOP/s", "frac": round(ach / info["peak_bf16_tflops"], 4),
+                            "traffic": traffic,
+                            "flops_per_launch": hw_flops, "avg_launch_ms": round(oz["gemm_kernel_ms"], 4)}
+        line["split_kernels_ms"] = round(oz["split_ms"], 4)
+    elif nat:
+        ach = flops / (nat["kernel_ms"] * 1e-3) / 1e12
+        line["roofline"] = {"kernel": "gemm_mfma_kernel<double>", "bound": "mfma", "achieved": round(ach, 2),
+                            "peak": round(info["peak_fp64_tflops"], 2), "unit": "TFLOP/s",
+                            "frac": round(ach / info["peak_fp64_tflops"], 4), "traffic": None}
+    if nat:
+        line["roofline_native"] = {"kernel": "gemm_mfma_kernel<double>", "bound": "mfma",
+                                   "achieved": round(flops / (nat["kernel_ms"] * 1e-3) / 1e12, 2),
+                                   "peak": round(info["peak_fp64_tflops"], 2), "unit": "TFLOP/s",
+                                   "frac": round(flops / (nat["kernel_ms"] * 1e-3) / 1e12 / info["peak_fp64_tflops"], 4)}
+    if table:
+        line["nslices_table"] = table
+    if cpu:
+        line["cpu_baseline"] = {"value": round(cpu["mt_tflops"], 5), "unit": "TFLOP/s", "cores": cpu["mt_threads"],
+                                "kind": "port",
+                                "sample": cpu["panel"] + "; the oracle's C restatement of jmul!'s fma recurrence "
+                                          "(register tile per pick_kernel_size, no packing, as the reference)",
+                                "single_thread_tflops": round(cpu["st_tflops"], 5),
+                                "seconds": round(cpu["mt_seconds"] + cpu["st_seconds"], 2)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

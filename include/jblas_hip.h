@@ -86,6 +86,17 @@ typedef struct jb_device_info {
 } jb_device_info;
 int jb_get_device_info(jb_device_info* out);
 
+/* ---- per-kernel timing (metrics; no counterpart in the reference, whose only
+ * timing is BenchmarkTools in test/runtests.jl:107) ------------------------------ */
+/* When on, every kernel launch is bracketed by HIP events on its stream (not
+ * inside graph capture).  Kernel names: "dgemm_mfma", "dgemm_recurrence",
+ * "sgemm_mfma", "ozaki_split", "slice_pair_gemm", "bf16x3_split", "bf16x3_gemm",
+ * "grouped_gemm". */
+int jb_kernel_timing_enable(int on);
+/* Total milliseconds and launch count of `kernel` since the last reset. */
+int jb_kernel_timing_read(const char* kernel, double* total_ms, int64_t* launches);
+int jb_kernel_timing_reset(void);
+
 /* ---- jmul! drop-in ------------------------------------------------------------ */
 /* D (M x P) = A (M x N) * X (N x P), column-major, leading dimensions M, N, M.
  * Per element: acc_1 = A[i,1]*X[1,j]; acc_n = fma(A[i,n], X[n,j], acc_{n-1}),

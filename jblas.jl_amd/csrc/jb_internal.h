@@ -46,6 +46,12 @@ hipError_t launch_fill_uniform(uint64_t seed, int64_t n, double* out, hipStream_
 hipError_t launch_fill_uniform_f32(uint64_t seed, int64_t n, float* out, hipStream_t s);
 hipError_t launch_fill_wide(uint64_t seed, int64_t n, int emin, int emax, double* out, hipStream_t s);
 
+// Per-kernel timing (off unless jb_kernel_timing_enable(1)): HIP events
+// recorded on the launch stream around one kernel launch.  Not recorded while
+// the stream is being captured into a graph.
+void timing_begin(hipStream_t s, const char* kernel);
+void timing_end(hipStream_t s, const char* kernel);
+
 // XCD-aware block swizzle: hardware places workgroup w on XCD (w % nxcd).
 // Returns a tile id such that each XCD owns a contiguous range of tile ids.
 __device__ __forceinline__ int xcd_remap(int w, int nblocks, int nxcd) {

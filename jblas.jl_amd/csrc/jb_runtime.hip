@@ -9,8 +9,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/jblas_hip.h"
 #include "jb_internal.h"
@@ -234,6 +236,55 @@ void finalize_locked() {
   r.inited = false;
 }
 
+// ---- per-kernel timing -------------------------------------------------------
+struct TimingRec {
+  std::string name;
+  hipEvent_t a = nullptr, b = nullptr;
+};
+static bool g_timing = false;
+static std::vector<TimingRec> g_pending;
+static std::map<std::string, std::pair<double, int64_t>> g_totals;
+
+static bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+void timing_begin(hipStream_t s, const char* kernel) {
+  if (!g_timing || capturing(s)) return;
+  TimingRec r;
+  r.name = kernel;
+  if (hipEventCreate(&r.a) != hipSuccess) return;
+  hipEventRecord(r.a, s);
+  g_pending.push_back(r);
+}
+
+void timing_end(hipStream_t s, const char* kernel) {
+  if (!g_timing || capturing(s)) return;
+  for (auto it = g_pending.rbegin(); it != g_pending.rend(); ++it) {
+    if (it->b == nullptr && it->name == kernel) {
+      if (hipEventCreate(&it->b) == hipSuccess) hipEventRecord(it->b, s);
+      return;
+    }
+  }
+}
+
+static void timing_collect() {
+  for (auto& r : g_pending) {
+    if (r.a && r.b && hipEventSynchronize(r.b) == hipSuccess) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+        auto& t = g_totals[r.name];
+        t.first += ms;
+        t.second += 1;
+      }
+    }
+    if (r.a) hipEventDestroy(r.a);
+    if (r.b) hipEventDestroy(r.b);
+  }
+  g_pending.clear();
+}
+
 int device_flag(int** out) {
   Runtime& r = rt();
   if (!r.flag) {
@@ -270,6 +321,33 @@ int jb_init(int device) {
 int jb_finalize(void) {
   std::lock_guard<std::mutex> lk(rt().mu);
   finalize_locked();
+  return JB_OK;
+}
+
+int jb_kernel_timing_enable(int on) {
+  std::lock_guard<std::mutex> lk(rt().mu);
+  tl_error.clear();
+  g_timing = on != 0;
+  return JB_OK;
+}
+
+int jb_kernel_timing_read(const char* kernel, double* total_ms, int64_t* launches) {
+  if (!kernel) return fail(-1, "kernel is NULL");
+  if (!total_ms) return fail(-2, "total_ms is NULL");
+  if (!launches) return fail(-3, "launches is NULL");
+  std::lock_guard<std::mutex> lk(rt().mu);
+  tl_error.clear();
+  timing_collect();
+  auto it = g_totals.find(kernel);
+  *total_ms = it == g_totals.end() ? 0.0 : it->second.first;
+  *launches = it == g_totals.end() ? 0 : it->second.second;
+  return JB_OK;
+}
+
+int jb_kernel_timing_reset(void) {
+  std::lock_guard<std::mutex> lk(rt().mu);
+  timing_collect();
+  g_totals.clear();
   return JB_OK;
 }
 

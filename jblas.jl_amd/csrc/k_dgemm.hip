@@ -319,9 +319,11 @@ static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
   auto B = static_cast<const T*>(g.B);
   auto C = static_cast<T*>(g.C);
   const T alpha = (T)g.alpha, beta = (T)g.beta;
+  const char* tname = sizeof(T) == 8 ? "dgemm_mfma" : "sgemm_mfma";
 #define JB_LAUNCH(TA, TB, V)                                                                           \
   hipLaunchKernelGGL((gemm_mfma_kernel<T, TA, TB, V>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, alpha, \
                      A, g.lda, B, g.ldb, beta, C, g.ldc, tiles_m, tiles_n)
+  timing_begin(s, tname);
   if (vec) {
     if (!g.ta && !g.tb) JB_LAUNCH(false, false, true);
     else if (!g.ta && g.tb) JB_LAUNCH(false, true, true);
@@ -334,6 +336,7 @@ static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
     else JB_LAUNCH(true, true, false);
   }
 #undef JB_LAUNCH
+  timing_end(s, tname);
   return hipGetLastError();
 }
 
@@ -349,11 +352,13 @@ hipError_t launch_dgemm_recurrence(const GemmArgs& g, hipStream_t s) {
 #define JB_LAUNCH(TA, TB)                                                                                 \
   hipLaunchKernelGGL((dgemm_recurrence_kernel<TA, TB>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, g.alpha, \
                      A, g.lda, B, g.ldb, g.beta, C, g.ldc, tiles_m, tiles_n)
+  timing_begin(s, "dgemm_recurrence");
   if (!g.ta && !g.tb) JB_LAUNCH(false, false);
   else if (!g.ta && g.tb) JB_LAUNCH(false, true);
   else if (g.ta && !g.tb) JB_LAUNCH(true, false);
   else JB_LAUNCH(true, true);
 #undef JB_LAUNCH
+  timing_end(s, "dgemm_recurrence");
   return hipGetLastError();
 }
 

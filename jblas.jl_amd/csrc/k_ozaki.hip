@@ -1,26 +1,33 @@
 // k_ozaki.hip -- split-BF16 emulated DGEMM and 3xBF16 SGEMM on gfx950.
 //
 // The reference has no emulated path (its GEMM is the FP64/FP32 register tile
-// of src/gemm.jl:244-348); this file implements the method BASELINE names:
+// of src/gemm.jl:244-348); this file implements the method of the emulated
+// DGEMM the drop-in adds:
 //
 //   1. absmax  (HBM-bound): per row of op(A) / column of op(B), the bit pattern
 //      of max|x| (an unsigned max of |x| bits orders finite < inf < NaN).
 //   2. split   (HBM-bound): E = frexp exponent of that max; x = ldexp(a, -E);
 //      balanced digits d_1 = rint(x*2^7), d_t = rint(r*2^8) of the running
-//      residual, |d| <= 128, each exact in BF16.  Slice planes are written
-//      k-contiguous and zero-padded: plane[t][row][k], rows padded to the block
-//      tile, k to the LDS stage -- the GEMM kernel needs no bounds checks.
-//   3. slice-pair GEMM (MFMA-bound): per 32x32 wave tile one FP32 accumulator
-//      set per level l = t+u (pairs t+u <= s+1), fed by
-//      v_mfma_f32_32x32x16_bf16.  Every kc k-values (kc*s <= 1024, so a level
-//      sum stays below 2^24 and is exact whatever the MFMA's internal order)
-//      the level sums are flushed into FP64 group accumulators,
-//      G_g = fma(S_l, 2^-8j, G_g) with l = 2+4g+j.  The epilogue combines the
-//      groups by TwoSum from the smallest, rescales by 2^(E_i+F_j-14) and
-//      applies alpha/beta.  These are the exact operations of the oracle's
-//      restatement, so the result is bit-reproducible.
+//      residual, |d| <= 128, each exact in BF16.  Slice planes are zero-padded
+//      and stored tile-contiguous: for a row block of RB rows (128 for op(A),
+//      64 for op(B)) and a k-stage of 16, the RB x 16 digits form one
+//      contiguous block (row-major, 32 bytes per row), so a GEMM stage is a
+//      handful of contiguous 1 KB pieces per plane.
+//   3. slice-pair GEMM (MFMA-bound): 4 waves per block, one per SIMD, each
+//      owning a 64 x 32 tile (two 32x32 MFMA blocks), one FP32 accumulator set
+//      per level l = t+u (pairs t+u <= s+1) on v_mfma_f32_32x32x16_bf16.
+//      Operand stages stream into a 3-slot LDS ring by direct global->LDS
+//      loads (global_load_lds_dwordx4), issued two stages ahead, one
+//      s_barrier per stage.  Every kc k-values (kc*s <= 1024, so a level sum
+//      stays below 2^24 and is exact whatever the MFMA's internal order) each
+//      level is flushed into an FP64 group accumulator right after its last
+//      MFMA of the stage, G_g = fma(S_l, 2^-8j, G_g) with l = 2+4g+j, so the
+//      flush overlaps the remaining MFMAs.  The epilogue combines the groups by
+//      TwoSum from the smallest, rescales by 2^(E_i+F_j-14) and applies
+//      alpha/beta.  These are the oracle restatement's operations in the same
+//      order, so the result is bit-reproducible.
 //   BF16X3 mode (FP32 inputs): per-element split a = a1 + a2 + a3 (bf16 round
-//   to nearest of the running residual, exact), products t+u <= 2+P, level 2
+//   to nearest of the running residual, exact), products t+u <= LMAX, level 2
 //   in one FP32 accumulator and the smaller levels in a second.
 #include <algorithm>
 
@@ -32,17 +39,28 @@ namespace {
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+#define JB_LDS __attribute__((address_space(3)))
 
-constexpr int ZNT = 512;            // 8 waves
-constexpr int ZBM = 128, ZBN = 64;  // block tile (4 x 2 waves of 32 x 32)
-constexpr int ZBK = 16;             // k per LDS stage = one MFMA k-step
-constexpr int ZROWS = ZBM + ZBN;    // rows of one slice-plane stage
-constexpr int ZPAD_K = 64;          // k padding of the planes (split tile width)
+constexpr int QBM = 128, QBN = 64;  // block tile (4/WB x 2 waves of 32*WB x 32)
+constexpr int QBK = 16;             // k per LDS stage = one MFMA k-step
+constexpr int QSLOTS = 3;           // LDS ring depth (prefetch distance 2)
+constexpr int QPAD_K = 64;          // k padding of the planes (split tile width)
+#ifndef OZ_WB
+#define OZ_WB 1  // emulated FP64: 8 waves of 32 x 32 (the FP64 group accumulators need the registers)
+#endif
+#ifndef X3_WB
+#define X3_WB 2  // 3xBF16: 4 waves of 64 x 32
+#endif
 
 __host__ __device__ constexpr int chunk_len(int s) {
   int kc = 1024;
   while (kc * s > 1024) kc >>= 1;
   return kc;
+}
+
+// Element offset of (row, k) in one tile-contiguous plane with row block RB.
+__device__ __forceinline__ int64_t plane_off(int64_t row, int64_t k, int RB, int64_t Kpad) {
+  return (row / RB) * (RB * Kpad) + (k / QBK) * (RB * QBK) + (row % RB) * QBK + (k % QBK);
 }
 
 __device__ __forceinline__ int exponent_of_bits(unsigned long long bits) {
@@ -56,8 +74,7 @@ __device__ __forceinline__ int exponent_of_bits(unsigned long long bits) {
 // 1. absmax: maxbits[r] = max_k bits(|X(r,k)|), X(r,k) = P[r*sr + k*sk].
 //    64 x 64 tiles, coalesced along whichever index is contiguous.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void absmax_kernel(const T* __restrict__ P, int64_t sr, int64_t sk, int64_t R,
+__global__ __launch_bounds__(256) void absmax_kernel(const double* __restrict__ P, int64_t sr, int64_t sk, int64_t R,
                                                      int64_t K, unsigned long long* __restrict__ maxbits) {
   __shared__ unsigned long long tile[64][65];
   const int t = threadIdx.x;
@@ -68,10 +85,7 @@ __global__ __launch_bounds__(256) void absmax_kernel(const T* __restrict__ P, in
     const int r = rows_contig ? (t % 64) : (t / 64 + 4 * j);
     const int k = rows_contig ? (t / 64 + 4 * j) : (t % 64);
     unsigned long long b = 0ull;
-    if (r0 + r < R && k0 + k < K) {
-      const double v = fabs((double)P[(r0 + r) * sr + (k0 + k) * sk]);
-      b = (unsigned long long)__double_as_longlong(v);
-    }
+    if (r0 + r < R && k0 + k < K) b = (unsigned long long)__double_as_longlong(fabs(P[(r0 + r) * sr + (k0 + k) * sk]));
     tile[r][k] = b;
   }
   __syncthreads();
@@ -88,19 +102,20 @@ __global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
 }
 
 __device__ __forceinline__ uint16_t bf16_bits_exact(double d) {
-  // d is an integer with |d| <= 256: exact in float, and the float's low 16 bits are zero.
+  // d is an integer with |d| <= 256: exact in float, whose low 16 bits are then zero.
   return (uint16_t)(__float_as_uint((float)d) >> 16);
 }
 
 // ---------------------------------------------------------------------------
-// 2a. FP64 split into S balanced-digit planes (+ exponents).
-//     Grid covers the padded Rpad x Kpad extent in 64 x 64 tiles.
+// 2a. FP64 split into S balanced-digit planes (+ exponents).  The grid covers
+//     the padded Rpad x Kpad extent in 64 x 64 tiles; RB divides by 64.
 // ---------------------------------------------------------------------------
 template <int S>
 __global__ __launch_bounds__(256) void split_f64_kernel(const double* __restrict__ P, int64_t sr, int64_t sk, int64_t R,
                                                         int64_t K, const unsigned long long* __restrict__ maxbits,
                                                         int* __restrict__ exps, uint16_t* __restrict__ planes,
-                                                        int64_t plane_stride, int64_t Kpad, int* __restrict__ flag) {
+                                                        int64_t plane_stride, int64_t Kpad, int RB,
+                                                        int* __restrict__ flag) {
   __shared__ double tile[64][65];
   const int t = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
@@ -129,6 +144,7 @@ __global__ __launch_bounds__(256) void split_f64_kernel(const double* __restrict
     double res[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) res[e] = ldexp(tile[row][kg + e], -E) * 128.0;
+    const int64_t off = plane_off(r0 + row, k0 + kg, RB, Kpad);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       uint32_t w[4];
@@ -143,8 +159,7 @@ __global__ __launch_bounds__(256) void split_f64_kernel(const double* __restrict
         res[e + 1] -= d1;
         w[e / 2] = (uint32_t)bf16_bits_exact(d0) | ((uint32_t)bf16_bits_exact(d1) << 16);
       }
-      u4 v = {w[0], w[1], w[2], w[3]};
-      *reinterpret_cast<u4*>(planes + s * plane_stride + (r0 + row) * Kpad + k0 + kg) = v;
+      *reinterpret_cast<u4*>(planes + s * plane_stride + off) = u4{w[0], w[1], w[2], w[3]};
     }
   }
 }
@@ -161,7 +176,7 @@ __device__ __forceinline__ float bf16_to_f(uint16_t b) { return __uint_as_float(
 
 __global__ __launch_bounds__(256) void split_f32_kernel(const float* __restrict__ P, int64_t sr, int64_t sk, int64_t R,
                                                         int64_t K, uint16_t* __restrict__ planes, int64_t plane_stride,
-                                                        int64_t Kpad) {
+                                                        int64_t Kpad, int RB) {
   __shared__ float tile[64][65];
   const int t = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
@@ -193,11 +208,10 @@ __global__ __launch_bounds__(256) void split_f32_kernel(const float* __restrict_
 #pragma unroll
       for (int s = 0; s < 3; ++s) w[s][e / 2] = (uint32_t)b[s][0] | ((uint32_t)b[s][1] << 16);
     }
+    const int64_t off = plane_off(r0 + row, k0 + kg, RB, Kpad);
 #pragma unroll
-    for (int s = 0; s < 3; ++s) {
-      u4 v = {w[s][0], w[s][1], w[s][2], w[s][3]};
-      *reinterpret_cast<u4*>(planes + s * plane_stride + (r0 + row) * Kpad + k0 + kg) = v;
-    }
+    for (int s = 0; s < 3; ++s)
+      *reinterpret_cast<u4*>(planes + s * plane_stride + off) = u4{w[s][0], w[s][1], w[s][2], w[s][3]};
   }
 }
 
@@ -206,8 +220,8 @@ __global__ __launch_bounds__(256) void split_f32_kernel(const float* __restrict_
 //    groups).  MODE 1: 3xBF16 FP32 (S = 3, pairs t+u <= LMAX).
 // ---------------------------------------------------------------------------
 struct PairArgs {
-  const uint16_t* pa;  // op(A) planes
-  const uint16_t* pb;  // op(B) planes
+  const uint16_t* pa;  // op(A) planes, row block 128
+  const uint16_t* pb;  // op(B) planes, row block 64
   int64_t Kpad, strideA, strideB;
   const int* eA;
   const int* eB;
@@ -224,136 +238,165 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
   s = x;
 }
 
-template <int MODE, int S, int LMAX>
-__global__ __launch_bounds__(ZNT, 1) void slice_pair_gemm_kernel(PairArgs p) {
-  // one LDS stage: S planes x ZROWS rows x 16 k (2 chunks of 8 bf16 per row)
-  constexpr int NCH = S * ZROWS * 2;             // 16-byte chunks per stage
-  constexpr int PER = (NCH + ZNT - 1) / ZNT;     // chunks per thread
-  constexpr int NL = (MODE == 0) ? S : 2;        // FP32 accumulator sets
-  constexpr int NG = (S + 3) / 4;                // FP64 groups (MODE 0)
-  constexpr int KC = chunk_len(S);
-  __shared__ __attribute__((aligned(16))) u4 lds[2][NCH];
+// s_waitcnt with only the vector-memory counter constrained (gfx9 encoding).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// Workgroup barrier without the fence __syncthreads() carries: the fence
+// would wait for every in-flight global->LDS load, draining the ring.  The
+// empty asm statements keep the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void block_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// WB = 32x32 MFMA blocks per wave along m: WB = 1 -> 8 waves (2 per SIMD) of
+// 32 x 32; WB = 2 -> 4 waves (1 per SIMD) of 64 x 32.  Both tile 128 x 64.
+template <int MODE, int S, int LMAX, int WB>
+__global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p) {
+  constexpr int NWAVES = 8 / WB, WM = 4 / WB;     // waves, waves along m
+  constexpr int PLANE_B = (QBM + QBN) * QBK * 2;  // bytes of one plane in one stage (6 KB)
+  constexpr int STAGE_B = S * PLANE_B;
+  constexpr int NWL = 6 * S;                      // 1 KB wave-loads per stage
+  constexpr int WL_MIN = NWL / NWAVES;            // fewest wave-loads any wave issues per stage
+  constexpr int NL = (MODE == 0) ? S : 2;         // FP32 accumulator sets
+  constexpr int NG = (S + 3) / 4;                 // FP64 groups (MODE 0)
+  constexpr int KC_ST = chunk_len(S) / QBK;       // stages per exact chunk
+  __shared__ __attribute__((aligned(1024))) char ring[QSLOTS * STAGE_B];
 
   const int nblocks = p.tiles_m * p.tiles_n;
   const int tile = xcd_remap(blockIdx.x, nblocks, 8);
   int tm, tn;
   tile_coords(tile, p.tiles_m, p.tiles_n, 8, tm, tn);
-  const int64_t m0 = (int64_t)tm * ZBM, n0 = (int64_t)tn * ZBN;
+  const int64_t m0 = (int64_t)tm * QBM, n0 = (int64_t)tn * QBN;
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-  const int wm0 = 32 * (wave % 4), wn0 = 32 * (wave / 4);
+  const int wm0 = 32 * WB * (wave % WM), wn0 = 32 * (wave / WM);
   const int l32 = lane % 32, hl = lane / 32;
 
-  // per-thread chunk sources (fixed for the whole k loop, advanced by 16 k per stage)
-  const uint16_t* src[PER];
-  int dst[PER];
-#pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int c = tid + ZNT * i;
-    const int cc = c < NCH ? c : 0;
-    const int t = cc / (ZROWS * 2), rem = cc % (ZROWS * 2), row = rem / 2, half = rem % 2;
-    src[i] = (row < ZBM) ? p.pa + t * p.strideA + (m0 + row) * p.Kpad + 8 * half
-                         : p.pb + t * p.strideB + (n0 + row - ZBM) * p.Kpad + 8 * half;
-    dst[i] = c < NCH ? c : -1;
-  }
-  u4 stg[PER];
-  auto load_stage = [&](int64_t k0) {
-#pragma unroll
-    for (int i = 0; i < PER; ++i)
-      if (dst[i] >= 0) stg[i] = *reinterpret_cast<const u4*>(src[i] + k0);
-  };
-  auto store_stage = [&](int buf) {
-#pragma unroll
-    for (int i = 0; i < PER; ++i)
-      if (dst[i] >= 0) lds[buf][dst[i]] = stg[i];
+  const uint16_t* gA = p.pa + (int64_t)tm * QBM * p.Kpad + lane * 8;
+  const uint16_t* gB = p.pb + (int64_t)tn * QBN * p.Kpad + lane * 8;
+  auto issue = [&](int ks, int slot) {
+    char* base = ring + slot * STAGE_B;
+    for (int j = wave; j < NWL; j += NWAVES) {
+      const uint16_t* src;
+      int loff;
+      if (j < 4 * S) {
+        const int t = j / 4, q = j % 4;
+        src = gA + t * p.strideA + (int64_t)ks * (QBM * QBK) + q * 512;
+        loff = t * PLANE_B + q * 1024;
+      } else {
+        const int jj = j - 4 * S, t = jj / 2, q = jj % 2;
+        src = gB + t * p.strideB + (int64_t)ks * (QBN * QBK) + q * 512;
+        loff = t * PLANE_B + QBM * QBK * 2 + q * 1024;
+      }
+      __builtin_amdgcn_global_load_lds((const void*)src, (JB_LDS void*)(base + loff), 16, 0, 0);
+    }
   };
 
-  f16v acc[NL];
+  f16v acc[NL][WB];
 #pragma unroll
   for (int l = 0; l < NL; ++l)
 #pragma unroll
-    for (int e = 0; e < 16; ++e) acc[l][e] = 0.0f;
-  double G[NG > 0 ? NG : 1][16];
+    for (int b = 0; b < WB; ++b)
 #pragma unroll
-  for (int g = 0; g < NG; ++g)
+      for (int e = 0; e < 16; ++e) acc[l][b][e] = 0.0f;
+  double G[MODE == 0 ? NG : 1][WB][16];
+  if (MODE == 0) {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) G[g][e] = 0.0;
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+      for (int b = 0; b < WB; ++b)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) G[g][b][e] = 0.0;
+  }
 
-  auto flush = [&]() {
-#pragma unroll
-    for (int l = 0; l < NL; ++l) {
-      const double sc = (l % 4 == 0) ? 1.0 : (l % 4 == 1) ? 0x1.0p-8 : (l % 4 == 2) ? 0x1.0p-16 : 0x1.0p-24;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        G[l / 4][e] = fma((double)acc[l][e], sc, G[l / 4][e]);
-        acc[l][e] = 0.0f;
-      }
-    }
-  };
-
-  const int nst = (int)(p.Kpad / ZBK);
-  constexpr int KC_ST = KC / ZBK;
-  load_stage(0);
-  store_stage(0);
-  __syncthreads();
+  const int nst = (int)(p.Kpad / QBK);
+  issue(0, 0);
+  if (nst > 1) issue(1, 1);
   for (int st = 0; st < nst; ++st) {
-    const int cur = st & 1;
-    if (st + 1 < nst) load_stage((int64_t)(st + 1) * ZBK);
-    const u4* L = lds[cur];
+    if (st + 1 < nst) wait_vmcnt<WL_MIN>();  // this stage landed; the next may be in flight
+    else wait_vmcnt<0>();
+    block_barrier();  // every wave's loads of stage st landed, every wave is done with st-1
+    if (st + 2 < nst) issue(st + 2, (st + 2) % QSLOTS);
+    const u4* L = reinterpret_cast<const u4*>(ring + (st % QSLOTS) * STAGE_B);
+    const bool chunk_end = (MODE == 0) && ((((st + 1) % KC_ST) == 0) || (st + 1 == nst));
     bf8 bfr[S];
 #pragma unroll
-    for (int u = 0; u < S; ++u) {
-      const u4 v = L[u * ZROWS * 2 + (ZBM + wn0 + l32) * 2 + hl];
-      bfr[u] = __builtin_bit_cast(bf8, v);
-    }
+    for (int u = 0; u < S; ++u) bfr[u] = __builtin_bit_cast(bf8, L[u * (PLANE_B / 16) + (QBM + wn0 + l32) * 2 + hl]);
 #pragma unroll
     for (int t = 0; t < S; ++t) {
-      const u4 v = L[t * ZROWS * 2 + (wm0 + l32) * 2 + hl];
-      const bf8 afr = __builtin_bit_cast(bf8, v);
+      bf8 a[WB];
+#pragma unroll
+      for (int b = 0; b < WB; ++b)
+        a[b] = __builtin_bit_cast(bf8, L[t * (PLANE_B / 16) + (wm0 + 32 * b + l32) * 2 + hl]);
 #pragma unroll
       for (int u = 0; u < S; ++u) {
         if (t + u + 2 > LMAX) continue;
         const int l = (MODE == 0) ? (t + u) : (t + u == 0 ? 0 : 1);
-        // X = op(B) plane (p -> n), Y = op(A) plane (q -> m): lane l holds column-major
-        // C[m0+wm0+l%32][n0+wn0+8(r/4)+4(l/32)+r%4] in item r.
-        acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bfr[u], afr, acc[l], 0, 0, 0);
+        // X = op(B) plane (p -> n), Y = op(A) plane (q -> m): item r of lane l holds
+        // C[m0+wm0+32b+l%32][n0+wn0+8(r/4)+4(l/32)+r%4].
+#pragma unroll
+        for (int b = 0; b < WB; ++b)
+          acc[l][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bfr[u], a[b], acc[l][b], 0, 0, 0);
+      }
+      if (MODE == 0 && chunk_end) {
+        // level t has received its last product of this chunk: flush it exactly
+        constexpr double kScale[4] = {1.0, 0x1.0p-8, 0x1.0p-16, 0x1.0p-24};
+#pragma unroll
+        for (int b = 0; b < WB; ++b)
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+            G[t / 4][b][e] = fma((double)acc[t][b][e], kScale[t % 4], G[t / 4][b][e]);
+            acc[t][b][e] = 0.0f;
+          }
       }
     }
-    if (st + 1 < nst) store_stage(cur ^ 1);
-    if (MODE == 0 && (((st + 1) % KC_ST) == 0 || st + 1 == nst)) flush();
-    __syncthreads();
   }
 
-  const int64_t m = m0 + wm0 + l32;
   if (MODE == 0) {
     double* C = static_cast<double*>(p.C);
-    const int ea = m < p.M ? p.eA[m] : 0;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int64_t n = n0 + wn0 + 8 * (e / 4) + 4 * hl + (e % 4);
-      if (m < p.M && n < p.N) {
-        double hi = 0.0, lo = 0.0;
+    for (int b = 0; b < WB; ++b) {
+      const int64_t m = m0 + wm0 + 32 * b + l32;
+      if (m >= p.M) continue;
+      const int ea = p.eA[m];
 #pragma unroll
-        for (int g = NG - 1; g >= 0; --g) {
-          double err;
-          two_sum(hi, ldexp(G[g][e], -32 * g), hi, err);
-          lo += err;
+      for (int e = 0; e < 16; ++e) {
+        const int64_t n = n0 + wn0 + 8 * (e / 4) + 4 * hl + (e % 4);
+        if (n < p.N) {
+          double hi = 0.0, lo = 0.0;
+#pragma unroll
+          for (int g = NG - 1; g >= 0; --g) {
+            double err;
+            two_sum(hi, ldexp(G[g][b][e], -32 * g), hi, err);
+            lo += err;
+          }
+          const double v = ldexp(hi + lo, ea + p.eB[n] - 14);
+          double c = p.alpha * v;
+          if (p.beta != 0.0) c = fma(p.beta, C[m + n * p.ldc], c);
+          C[m + n * p.ldc] = c;
         }
-        const double v = ldexp(hi + lo, ea + p.eB[n] - 14);
-        double c = p.alpha * v;
-        if (p.beta != 0.0) c = fma(p.beta, C[m + n * p.ldc], c);
-        C[m + n * p.ldc] = c;
       }
     }
   } else {
     float* C = static_cast<float*>(p.C);
     const float alpha = (float)p.alpha, beta = (float)p.beta;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int64_t n = n0 + wn0 + 8 * (e / 4) + 4 * hl + (e % 4);
-      if (m < p.M && n < p.N) {
-        float c = alpha * (acc[0][e] + acc[1][e]);
-        if (beta != 0.0f) c = fmaf(beta, C[m + n * p.ldc], c);
-        C[m + n * p.ldc] = c;
+    for (int b = 0; b < WB; ++b) {
+      const int64_t m = m0 + wm0 + 32 * b + l32;
+      if (m >= p.M) continue;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int64_t n = n0 + wn0 + 8 * (e / 4) + 4 * hl + (e % 4);
+        if (n < p.N) {
+          float c = alpha * (acc[0][b][e] + acc[1][b][e]);
+          if (beta != 0.0f) c = fmaf(beta, C[m + n * p.ldc], c);
+          C[m + n * p.ldc] = c;
+        }
       }
     }
   }
@@ -371,9 +414,9 @@ struct Layout {
 
 Layout make_layout(int64_t m, int64_t n, int64_t k, int s) {
   Layout L;
-  L.Mpad = rup(std::max<int64_t>(m, 1), 128);
-  L.Npad = rup(std::max<int64_t>(n, 1), 128);
-  L.Kpad = rup(std::max<int64_t>(k, 1), ZPAD_K);
+  L.Mpad = rup(std::max<int64_t>(m, 1), QBM);
+  L.Npad = rup(std::max<int64_t>(n, 1), QBN);
+  L.Kpad = rup(std::max<int64_t>(k, 1), QPAD_K);
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   size_t off = 0;
   L.offA = off;
@@ -404,31 +447,18 @@ inline void strides_b(bool tb, int64_t ldb, int64_t& sr, int64_t& sk) {
 }
 
 template <int S>
-hipError_t launch_split_f64(const double* P, int64_t sr, int64_t sk, int64_t R, int64_t K, int64_t Rpad, int64_t Kpad,
-                            unsigned long long* mb, int* ex, uint16_t* planes, int* flag, hipStream_t st) {
-  const dim3 g((unsigned)(Rpad / 64), (unsigned)(Kpad / 64));
-  hipLaunchKernelGGL(zero_u64_kernel, dim3(std::min<int64_t>((Rpad + 255) / 256, 1024)), dim3(256), 0, st, mb, Rpad);
-  const dim3 gr((unsigned)((R + 63) / 64), (unsigned)((K + 63) / 64));
-  hipLaunchKernelGGL(absmax_kernel<double>, gr, dim3(256), 0, st, P, sr, sk, R, K, mb);
-  hipLaunchKernelGGL(split_f64_kernel<S>, g, dim3(256), 0, st, P, sr, sk, R, K, mb, ex, planes, Rpad * Kpad, Kpad, flag);
-  return hipGetLastError();
+void launch_split_f64(const double* P, int64_t sr, int64_t sk, int64_t R, int64_t K, int64_t Rpad, int64_t Kpad, int RB,
+                      unsigned long long* mb, int* ex, uint16_t* planes, int* flag, hipStream_t st) {
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)std::min<int64_t>((Rpad + 255) / 256, 1024)), dim3(256), 0, st,
+                     mb, Rpad);
+  hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)((R + 63) / 64), (unsigned)((K + 63) / 64)), dim3(256), 0, st, P,
+                     sr, sk, R, K, mb);
+  hipLaunchKernelGGL(split_f64_kernel<S>, dim3((unsigned)(Rpad / 64), (unsigned)(Kpad / 64)), dim3(256), 0, st, P, sr,
+                     sk, R, K, mb, ex, planes, Rpad * Kpad, Kpad, RB, flag);
 }
 
-template <int S>
-hipError_t run_ozaki(const GemmArgs& g, const Layout& L, char* ws, int* flag, hipStream_t st) {
-  int64_t sr, sk;
-  auto* mbA = reinterpret_cast<unsigned long long*>(ws + L.offMA);
-  auto* mbB = reinterpret_cast<unsigned long long*>(ws + L.offMB);
-  auto* eA = reinterpret_cast<int*>(ws + L.offEA);
-  auto* eB = reinterpret_cast<int*>(ws + L.offEB);
-  auto* pA = reinterpret_cast<uint16_t*>(ws + L.offA);
-  auto* pB = reinterpret_cast<uint16_t*>(ws + L.offB);
-  strides_a(g.ta, g.lda, sr, sk);
-  hipError_t e = launch_split_f64<S>(static_cast<const double*>(g.A), sr, sk, g.m, g.k, L.Mpad, L.Kpad, mbA, eA, pA, flag, st);
-  if (e != hipSuccess) return e;
-  strides_b(g.tb, g.ldb, sr, sk);
-  e = launch_split_f64<S>(static_cast<const double*>(g.B), sr, sk, g.n, g.k, L.Npad, L.Kpad, mbB, eB, pB, flag, st);
-  if (e != hipSuccess) return e;
+PairArgs pair_args(const GemmArgs& g, const Layout& L, const uint16_t* pA, const uint16_t* pB, const int* eA,
+                   const int* eB) {
   PairArgs p;
   p.pa = pA;
   p.pb = pB;
@@ -443,9 +473,32 @@ hipError_t run_ozaki(const GemmArgs& g, const Layout& L, char* ws, int* flag, hi
   p.beta = g.beta;
   p.C = g.C;
   p.ldc = g.ldc;
-  p.tiles_m = (int)((g.m + ZBM - 1) / ZBM);
-  p.tiles_n = (int)((g.n + ZBN - 1) / ZBN);
-  hipLaunchKernelGGL((slice_pair_gemm_kernel<0, S, S + 1>), dim3(p.tiles_m * p.tiles_n), dim3(ZNT), 0, st, p);
+  p.tiles_m = (int)(L.Mpad / QBM);
+  p.tiles_n = (int)(L.Npad / QBN);
+  return p;
+}
+
+template <int S>
+hipError_t run_ozaki(const GemmArgs& g, const Layout& L, char* ws, int* flag, hipStream_t st) {
+  int64_t sr, sk;
+  auto* mbA = reinterpret_cast<unsigned long long*>(ws + L.offMA);
+  auto* mbB = reinterpret_cast<unsigned long long*>(ws + L.offMB);
+  auto* eA = reinterpret_cast<int*>(ws + L.offEA);
+  auto* eB = reinterpret_cast<int*>(ws + L.offEB);
+  auto* pA = reinterpret_cast<uint16_t*>(ws + L.offA);
+  auto* pB = reinterpret_cast<uint16_t*>(ws + L.offB);
+  timing_begin(st, "ozaki_split");
+  strides_a(g.ta, g.lda, sr, sk);
+  launch_split_f64<S>(static_cast<const double*>(g.A), sr, sk, g.m, g.k, L.Mpad, L.Kpad, QBM, mbA, eA, pA, flag, st);
+  strides_b(g.tb, g.ldb, sr, sk);
+  launch_split_f64<S>(static_cast<const double*>(g.B), sr, sk, g.n, g.k, L.Npad, L.Kpad, QBN, mbB, eB, pB, flag, st);
+  timing_end(st, "ozaki_split");
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const PairArgs p = pair_args(g, L, pA, pB, eA, eB);
+  timing_begin(st, "slice_pair_gemm");
+  hipLaunchKernelGGL((slice_pair_gemm_kernel<0, S, S + 1, OZ_WB>), dim3(p.tiles_m * p.tiles_n), dim3(512 / OZ_WB), 0, st, p);
+  timing_end(st, "slice_pair_gemm");
   return hipGetLastError();
 }
 
@@ -477,34 +530,23 @@ hipError_t launch_sgemm_bf16x3(const GemmArgs& g, int nprod, void* ws, hipStream
   auto* pA = reinterpret_cast<uint16_t*>(w + L.offA);
   auto* pB = reinterpret_cast<uint16_t*>(w + L.offB);
   int64_t sr, sk;
+  timing_begin(st, "bf16x3_split");
   strides_a(g.ta, g.lda, sr, sk);
   hipLaunchKernelGGL(split_f32_kernel, dim3((unsigned)(L.Mpad / 64), (unsigned)(L.Kpad / 64)), dim3(256), 0, st,
-                     static_cast<const float*>(g.A), sr, sk, g.m, g.k, pA, L.Mpad * L.Kpad, L.Kpad);
+                     static_cast<const float*>(g.A), sr, sk, g.m, g.k, pA, L.Mpad * L.Kpad, L.Kpad, QBM);
   strides_b(g.tb, g.ldb, sr, sk);
   hipLaunchKernelGGL(split_f32_kernel, dim3((unsigned)(L.Npad / 64), (unsigned)(L.Kpad / 64)), dim3(256), 0, st,
-                     static_cast<const float*>(g.B), sr, sk, g.n, g.k, pB, L.Npad * L.Kpad, L.Kpad);
+                     static_cast<const float*>(g.B), sr, sk, g.n, g.k, pB, L.Npad * L.Kpad, L.Kpad, QBN);
+  timing_end(st, "bf16x3_split");
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  PairArgs p;
-  p.pa = pA;
-  p.pb = pB;
-  p.Kpad = L.Kpad;
-  p.strideA = L.Mpad * L.Kpad;
-  p.strideB = L.Npad * L.Kpad;
-  p.eA = nullptr;
-  p.eB = nullptr;
-  p.M = g.m;
-  p.N = g.n;
-  p.alpha = g.alpha;
-  p.beta = g.beta;
-  p.C = g.C;
-  p.ldc = g.ldc;
-  p.tiles_m = (int)((g.m + ZBM - 1) / ZBM);
-  p.tiles_n = (int)((g.n + ZBN - 1) / ZBN);
+  const PairArgs p = pair_args(g, L, pA, pB, nullptr, nullptr);
   const dim3 grid(p.tiles_m * p.tiles_n);
-  if (nprod == 3) hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 3>), grid, dim3(ZNT), 0, st, p);
-  else if (nprod == 6) hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 4>), grid, dim3(ZNT), 0, st, p);
-  else hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 6>), grid, dim3(ZNT), 0, st, p);
+  timing_begin(st, "bf16x3_gemm");
+  if (nprod == 3) hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 3, X3_WB>), grid, dim3(512 / X3_WB), 0, st, p);
+  else if (nprod == 6) hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 4, X3_WB>), grid, dim3(512 / X3_WB), 0, st, p);
+  else hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 6, X3_WB>), grid, dim3(512 / X3_WB), 0, st, p);
+  timing_end(st, "bf16x3_gemm");
   return hipGetLastError();
 }
 

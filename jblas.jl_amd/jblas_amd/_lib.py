@@ -64,6 +64,9 @@ SIGNATURES = {
     "jb_last_error": (C.c_char_p, []),
     "jb_version": (_int, []),
     "jb_get_device_info": (_int, [C.POINTER(DeviceInfo)]),
+    "jb_kernel_timing_enable": (_int, [_int]),
+    "jb_kernel_timing_read": (_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(_i64)]),
+    "jb_kernel_timing_reset": (_int, []),
     "jb_jmul": (_int, [_i64, _i64, _i64, _p, _p, _p]),
     "jb_jmul_dev": (_int, [_i64, _i64, _i64, _p, _p, _p, _p]),
     "jb_dgemm": (_int, [_ch, _ch, _i64, _i64, _i64, _dbl, _p, _i64, _p, _i64, _dbl, _p, _i64]),
@@ -135,6 +138,21 @@ def init(device: int = 0) -> None:
 
 def finalize() -> None:
     call("jb_finalize")
+
+
+def kernel_timing(on: bool) -> None:
+    call("jb_kernel_timing_enable", 1 if on else 0)
+
+
+def kernel_time(kernel: str):
+    """(total milliseconds, launches) of `kernel` since the last reset."""
+    ms, n = C.c_double(), C.c_int64()
+    call("jb_kernel_timing_read", kernel.encode(), C.byref(ms), C.byref(n))
+    return ms.value, n.value
+
+
+def kernel_timing_reset() -> None:
+    call("jb_kernel_timing_reset")
 
 
 def device_info() -> dict:

@@ -186,24 +186,89 @@ static inline double opB(char t, const double* B, int64_t ldb, int64_t p, int64_
  * product and the epilogue c = alpha*acc, then c = fma(beta, c_old, c) when
  * beta != 0 (beta == 0 never reads C).  alpha == 0 or k == 0 gives beta*C.
  */
+/*
+ * Packed copies of op(A) (m x k) and op(B) (k x n), both column-major and
+ * contiguous, so that the chains of 8 consecutive rows run side by side in
+ * one vector (the per-element operation order is unchanged).
+ */
+static double* pack_op(char t, const double* X, int64_t ldx, int64_t rows, int64_t cols, int nthreads) {
+  double* P = (double*)malloc(sizeof(double) * (size_t)(rows * cols + 1));
+  if (!P) return NULL;
+  const int tr = !(t == 'N' || t == 'n');
+  (void)nthreads;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t c = 0; c < cols; ++c)
+    for (int64_t r = 0; r < rows; ++r) P[r + c * rows] = tr ? X[c + r * ldx] : X[r + c * ldx];
+  return P;
+}
+
+/* Chains for rows [i0, i0+nr) of column j over k in [c0, c1): acc[r] for each row. */
+static inline void chain_block(const double* Ap, const double* Bp, int64_t m, int64_t i0, int nr, int64_t j,
+                               int64_t k, int64_t c0, int64_t c1, double* acc) {
+  const double* b = Bp + j * k;
+  if (nr == 8) {
+    double a8[8];
+    for (int r = 0; r < 8; ++r) a8[r] = Ap[i0 + r + c0 * m] * b[c0];
+    for (int64_t p = c0 + 1; p < c1; ++p) {
+      const double* a = Ap + i0 + p * m;
+      const double x = b[p];
+      for (int r = 0; r < 8; ++r) a8[r] = fma(a[r], x, a8[r]);
+    }
+    for (int r = 0; r < 8; ++r) acc[r] = a8[r];
+  } else {
+    for (int r = 0; r < nr; ++r) {
+      double s = Ap[i0 + r + c0 * m] * b[c0];
+      for (int64_t p = c0 + 1; p < c1; ++p) s = fma(Ap[i0 + r + p * m], b[p], s);
+      acc[r] = s;
+    }
+  }
+}
+
+/*
+ * Shared body of the BLAS-form recurrence: k cut into chunks of kc (kc >= k:
+ * one chain), chunk results summed in chunk order, then the epilogue.
+ */
+static void gemm_recurrence_impl(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+                                 const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                                 double* C, int64_t ldc, int64_t kc, int nthreads) {
+  if (alpha == 0.0 || k == 0) {
+    for (int64_t j = 0; j < n; ++j)
+      for (int64_t i = 0; i < m; ++i) C[i + j * ldc] = (beta == 0.0) ? 0.0 : beta * C[i + j * ldc];
+    return;
+  }
+  if (kc < 1 || kc > k) kc = k;
+  double* Ap = pack_op(ta, A, lda, m, k, nthreads);
+  double* Bp = pack_op((tb == 'N' || tb == 'n') ? 'N' : 'T', B, ldb, k, n, nthreads);
+  if (!Ap || !Bp) {
+    free(Ap);
+    free(Bp);
+    return;
+  }
+  (void)nthreads;
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t j = 0; j < n; ++j)
+    for (int64_t i0 = 0; i0 < m; i0 += 8) {
+      const int nr = (int)(m - i0 < 8 ? m - i0 : 8);
+      double s[8] = {0, 0, 0, 0, 0, 0, 0, 0}, acc[8];
+      for (int64_t c0 = 0; c0 < k; c0 += kc) {
+        const int64_t c1 = c0 + kc < k ? c0 + kc : k;
+        chain_block(Ap, Bp, m, i0, nr, j, k, c0, c1, acc);
+        for (int r = 0; r < nr; ++r) s[r] = (c0 == 0) ? acc[r] : s[r] + acc[r];
+      }
+      for (int r = 0; r < nr; ++r) {
+        double c = alpha * s[r];
+        if (beta != 0.0) c = fma(beta, C[i0 + r + j * ldc], c);
+        C[i0 + r + j * ldc] = c;
+      }
+    }
+  free(Ap);
+  free(Bp);
+}
+
 ORC_API void orc_gemm_recurrence(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
                                  const double* A, int64_t lda, const double* B, int64_t ldb,
                                  double beta, double* C, int64_t ldc, int nthreads) {
-  (void)nthreads;
-#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
-  for (int64_t j = 0; j < n; ++j)
-    for (int64_t i = 0; i < m; ++i) {
-      double c;
-      if (alpha == 0.0 || k == 0) {
-        c = (beta == 0.0) ? 0.0 : beta * C[i + j * ldc];
-      } else {
-        double acc = opA(ta, A, lda, i, 0) * opB(tb, B, ldb, 0, j);
-        for (int64_t p = 1; p < k; ++p) acc = fma(opA(ta, A, lda, i, p), opB(tb, B, ldb, p, j), acc);
-        c = alpha * acc;
-        if (beta != 0.0) c = fma(beta, C[i + j * ldc], c);
-      }
-      C[i + j * ldc] = c;
-    }
+  gemm_recurrence_impl(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, k, nthreads);
 }
 
 /* The same recurrence on sampled entries (ii[s], jj[s]); c0[s] is the old C entry. */
@@ -226,6 +291,18 @@ ORC_API void orc_gemm_recurrence_sample(char ta, char tb, int64_t k, double alph
     }
     out[s] = c;
   }
+}
+
+/*
+ * Split-K form of the recurrence (the grouped launch's deterministic split-K):
+ * k is cut into chunks [q*kc, min(k, (q+1)*kc)); each chunk is its own chain
+ * (first product, then fma in increasing k), and the chunk results are summed
+ * in chunk order, s = p_0; s = s + p_q.  kc >= k gives the plain recurrence.
+ */
+ORC_API void orc_gemm_recurrence_chunked(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+                                         const double* A, int64_t lda, const double* B, int64_t ldb,
+                                         double beta, double* C, int64_t ldc, int64_t kc, int nthreads) {
+  gemm_recurrence_impl(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kc, nthreads);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -55,6 +55,8 @@ def lib():
     L.orc_gemm_recurrence.argtypes = [C.c_char, C.c_char, i64, i64, i64, dbl, p, i64, p, i64, dbl, p, i64, C.c_int]
     L.orc_gemm_recurrence_sample.argtypes = [C.c_char, C.c_char, i64, dbl, p, i64, p, i64, dbl, p, i64, p, p, p,
                                              C.c_int]
+    L.orc_gemm_recurrence_chunked.argtypes = [C.c_char, C.c_char, i64, i64, i64, dbl, p, i64, p, i64, dbl, p, i64,
+                                              i64, C.c_int]
     L.orc_gemm_dot2.argtypes = [C.c_char, C.c_char, i64, i64, i64, dbl, p, i64, p, i64, dbl, p, i64, C.c_int]
     L.orc_gemm_dot2_sample.argtypes = [C.c_char, C.c_char, i64, dbl, p, i64, p, i64, dbl, p, i64, p, p, p, C.c_int]
     L.orc_sgemm_dot2.argtypes = [C.c_char, C.c_char, i64, i64, i64, dbl, p, i64, p, i64, dbl, p, i64, p, i64, C.c_int]
@@ -146,6 +148,16 @@ def gemm_recurrence(ta, tb, alpha, A, B, beta, C, nthreads=None) -> np.ndarray:
     m, n, k = _dims(ta, tb, A, B)
     lib().orc_gemm_recurrence(ta.encode(), tb.encode(), m, n, k, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B),
                               beta, _ptr(C), _ld(C), nthreads or nthreads_default())
+    return C
+
+
+def gemm_recurrence_chunked(ta, tb, alpha, A, B, beta, C, kc, nthreads=None) -> np.ndarray:
+    """The recurrence with k cut into chunks of kc, chunk results summed in order."""
+    A, B = _f(A), _f(B)
+    C = np.array(C, dtype=np.float64, order="F", copy=True)
+    m, n, k = _dims(ta, tb, A, B)
+    lib().orc_gemm_recurrence_chunked(ta.encode(), tb.encode(), m, n, k, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B),
+                                      beta, _ptr(C), _ld(C), int(kc), nthreads or nthreads_default())
     return C
 
 

@@ -1,0 +1,361 @@
+"""GPU parity of the grouped (batched) GEMM, its hipGraph plans, the
+autotuner and the gemm! pullback (rrule), through the C ABI.
+
+Bars:
+  * FP64 grouped: BIT-EXACT against the oracle's split-K restatement of the
+    reference recurrence (oracle.gemm_recurrence_chunked with the k-chunk the
+    library chose for that problem) -- each split is an fma chain in k order
+    (f64 MFMA) and the partials are summed in split order.
+  * FP32 (native MFMA and 3xBF16): |C - C*| <= 2*K*2^-24 (|alpha||A||B| + |beta||C0|)
+    componentwise against the FP64 oracle.
+  * graph replays are bit-identical to the eager launch, replay after replay.
+  * rrule: closed form dA = W B^T, dB = A^T W; FP64 central differences
+    (h = 1e-6) on a 64^3 problem agree to 1e-7 relative (the loss is linear, so
+    the difference quotient is exact up to rounding ~ u*|L|/h).
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import jblas_amd as jb
+from jblas_amd import generators as G
+from jblas_amd.grouped import GemmDesc
+from oracle import oracle as O
+
+from conftest import to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+U32 = 2.0 ** -24
+
+
+def dev_uniform(torch, rows, cols, seed, dtype):
+    t = torch.empty((cols, rows), dtype=dtype, device="cuda")
+    jb.fill_uniform_(t, seed)
+    return t.T  # column-major rows x cols, column-major fill of the stream
+
+
+def make_group(torch, count, seed, dtype, ops=("N", "N"), alpha=1.0, beta=0.0):
+    shapes = G.loguniform_shapes(seed, count)
+    probs = []
+    for i, (m, n, k) in enumerate(shapes):
+        ta, tb = ops if isinstance(ops, tuple) else ops[i % len(ops)]
+        A = dev_uniform(torch, m if ta == "N" else k, k if ta == "N" else m, 1000 + 3 * i, dtype)
+        B = dev_uniform(torch, k if tb == "N" else n, n if tb == "N" else k, 1001 + 3 * i, dtype)
+        C = dev_uniform(torch, m, n, 1002 + 3 * i, dtype)
+        probs.append((ta, tb, int(m), int(n), int(k), A, B, C))
+    descs = [GemmDesc(ta, tb, alpha, A, B, beta, C) for (ta, tb, m, n, k, A, B, C) in probs]
+    return probs, descs
+
+
+def test_grouped_f64_c4_bitexact(gpu):
+    """Config: 1024 problems, m,n,k log-uniform in [16, 2048] (multiples of 8), FP64."""
+    torch = gpu
+    probs, descs = make_group(torch, 1024, 52, torch.float64)
+    C0 = [to_host(p[7]) for p in probs[::8]]
+    jb.gemm_grouped_("f64", descs)
+    torch.cuda.synchronize()
+    for idx, (ta, tb, m, n, k, A, B, C) in enumerate(probs[::8]):
+        tile, sk = jb.tune_lookup("f64", m, n, k)
+        ref = O.gemm_recurrence_chunked(ta, tb, 1.0, to_host(A), to_host(B), 0.0, C0[idx], jb.split_chunk(k, sk))
+        assert np.array_equal(to_host(C), ref), (m, n, k, tile, sk)
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_grouped_f64_ops_alpha_beta(gpu, ta, tb):
+    torch = gpu
+    probs, descs = make_group(torch, 48, 60 + ord(ta) + 2 * ord(tb), torch.float64, ops=(ta, tb), alpha=1.5,
+                              beta=0.5)
+    C0 = [to_host(p[7]) for p in probs]
+    jb.gemm_grouped_("f64", descs)
+    torch.cuda.synchronize()
+    for idx, (ta_, tb_, m, n, k, A, B, C) in enumerate(probs):
+        _, sk = jb.tune_lookup("f64", m, n, k)
+        ref = O.gemm_recurrence_chunked(ta, tb, 1.5, to_host(A), to_host(B), 0.5, C0[idx], jb.split_chunk(k, sk))
+        assert np.array_equal(to_host(C), ref), (m, n, k, sk)
+
+
+def test_grouped_forced_splitk_bitexact(gpu):
+    """Small outputs with long k take split-K; the combine is deterministic."""
+    torch = gpu
+    shapes = [(16, 16, 2048), (24, 40, 1536), (64, 64, 2000), (8, 8, 4096)]
+    probs = []
+    for i, (m, n, k) in enumerate(shapes):
+        A = dev_uniform(torch, m, k, 10 + i, torch.float64)
+        B = dev_uniform(torch, k, n, 20 + i, torch.float64)
+        C = dev_uniform(torch, m, n, 30 + i, torch.float64)
+        probs.append((m, n, k, A, B, C))
+    C0 = [to_host(p[5]) for p in probs]
+    choices = [jb.tune_lookup("f64", m, n, k) for (m, n, k, *_r) in probs]
+    assert any(sk > 1 for _, sk in choices), choices
+    for rep in range(2):  # the combine must give the same bits every time
+        for i, p in enumerate(probs):
+            p[5].copy_(to_dev(torch, C0[i]))
+        jb.gemm_grouped_("f64", [GemmDesc("N", "N", 1.0, A, B, 1.0, C) for (m, n, k, A, B, C) in probs])
+        torch.cuda.synchronize()
+        for i, (m, n, k, A, B, C) in enumerate(probs):
+            ref = O.gemm_recurrence_chunked("N", "N", 1.0, to_host(A), to_host(B), 1.0, C0[i],
+                                            jb.split_chunk(k, choices[i][1]))
+            assert np.array_equal(to_host(C), ref), (m, n, k, choices[i])
+
+
+def test_grouped_plan_graph_replays(gpu):
+    torch = gpu
+    probs, descs = make_group(torch, 96, 53, torch.float64, alpha=1.0, beta=1.0)
+    C0 = [p[7].clone() for p in probs]
+    jb.gemm_grouped_("f64", descs)  # eager reference: C1 = A*B + C0
+    torch.cuda.synchronize()
+    C1 = [p[7].clone() for p in probs]
+    plan = jb.GroupedPlan("f64", descs)
+    for rep in range(3):
+        for i, p in enumerate(probs):
+            p[7].copy_(C0[i])
+        junk = torch.randn(512, 512, device="cuda") @ torch.randn(512, 512, device="cuda")  # eager work between
+        plan.launch()
+        torch.cuda.synchronize()
+        del junk
+        for i, p in enumerate(probs):
+            assert torch.equal(p[7], C1[i]), (rep, i)
+    plan.destroy()
+
+
+@pytest.mark.parametrize("method", ["f32", "f32_bf16x3"])
+def test_grouped_f32_tolerance(gpu, method):
+    torch = gpu
+    probs, descs = make_group(torch, 256, 54, torch.float32, ops=[("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")],
+                              alpha=1.5, beta=0.5)
+    C0 = [to_host(p[7]) for p in probs[::8]]
+    jb.gemm_grouped_(method, descs)
+    torch.cuda.synchronize()
+    for idx, (ta, tb, m, n, k, A, B, C) in enumerate(probs[::8]):
+        Ah, Bh = to_host(A), to_host(B)
+        ref = O.sgemm_dot2(ta, tb, 1.5, Ah, Bh, 0.5, C0[idx])
+        Ao = Ah if ta == "N" else Ah.T
+        Bo = Bh if tb == "N" else Bh.T
+        absab = 1.5 * (np.abs(Ao).astype(float) @ np.abs(Bo).astype(float)) + 0.5 * np.abs(C0[idx])
+        err = np.max(np.abs(to_host(C).astype(float) - ref) / absab)
+        assert err <= 2 * k * U32, (method, m, n, k, err)
+
+
+def test_grouped_edge_cases(gpu):
+    torch = gpu
+    A = dev_uniform(torch, 40, 30, 1, torch.float64)
+    B = dev_uniform(torch, 30, 20, 2, torch.float64)
+    C = dev_uniform(torch, 40, 20, 3, torch.float64)
+    Cz = dev_uniform(torch, 40, 20, 4, torch.float64)
+    C0, Cz0 = to_host(C), to_host(Cz)
+    empty_A = torch.empty((0, 40), dtype=torch.float64, device="cuda").T  # 40 x 0
+    empty_B = torch.empty((20, 0), dtype=torch.float64, device="cuda").T  # 0 x 20
+    Ck = dev_uniform(torch, 40, 20, 5, torch.float64)
+    Ck0 = to_host(Ck)
+    big = dev_uniform(torch, 64, 64, 6, torch.float64)
+    descs = [
+        GemmDesc("N", "N", 0.0, A, B, 0.5, C),          # alpha = 0: beta*C
+        GemmDesc("N", "N", 2.0, empty_A, empty_B, -1.0, Ck),  # k = 0
+        GemmDesc("N", "N", 1.0, A, B, 0.0, Cz),         # beta = 0
+        GemmDesc("N", "N", 1.0, big[:10, :7], big[20:27, 30:39], 1.0, big[40:50, 50:59]),  # strided views
+    ]
+    big0 = to_host(big)
+    jb.gemm_grouped_("f64", descs)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(C), 0.5 * C0)
+    assert np.array_equal(to_host(Ck), -Ck0)
+    assert np.array_equal(to_host(Cz), O.gemm_recurrence("N", "N", 1.0, to_host(A), to_host(B), 0.0, Cz0))
+    out = to_host(big)
+    ref = O.gemm_recurrence("N", "N", 1.0, big0[:10, :7], big0[20:27, 30:39], 1.0, big0[40:50, 50:59])
+    assert np.array_equal(out[40:50, 50:59], ref)
+    mask = np.ones_like(out, dtype=bool)
+    mask[40:50, 50:59] = False
+    assert np.array_equal(out[mask], big0[mask])
+
+
+def test_autotune_roundtrip(gpu):
+    torch = gpu
+    probs, descs = make_group(torch, 64, 55, torch.float64)
+    jb.tune_reset()
+    before = [jb.tune_lookup("f64", m, n, k) for (_, _, m, n, k, *_r) in probs]
+    jb.autotune("f64", descs, reps=1)
+    tuned = [jb.tune_lookup("f64", m, n, k) for (_, _, m, n, k, *_r) in probs]
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "tune.json")
+        jb.tune_save(path)
+        text = open(path).read()
+        assert "jblas-tune-1" in text
+        jb.tune_reset()
+        assert [jb.tune_lookup("f64", m, n, k) for (_, _, m, n, k, *_r) in probs] == before
+        jb.tune_load(path)
+        assert [jb.tune_lookup("f64", m, n, k) for (_, _, m, n, k, *_r) in probs] == tuned
+        with open(path, "w") as f:
+            f.write("not a table")
+        with pytest.raises(jb.JBError):
+            jb.tune_load(path)
+    # the tuned table still gives bit-exact results
+    C0 = [to_host(p[7]) for p in probs]
+    jb.gemm_grouped_("f64", descs)
+    torch.cuda.synchronize()
+    for idx, (ta, tb, m, n, k, A, B, C) in enumerate(probs[::4]):
+        _, sk = jb.tune_lookup("f64", m, n, k)
+        ref = O.gemm_recurrence_chunked("N", "N", 1.0, to_host(A), to_host(B), 0.0, C0[4 * idx],
+                                        jb.split_chunk(k, sk))
+        assert np.array_equal(to_host(C), ref)
+    jb.tune_reset()
+
+
+def test_grouped_fault_injection(gpu):
+    torch = gpu
+    probs, descs = make_group(torch, 4, 56, torch.float64)
+    os.environ["JB_FAULT_GRAPH"] = "1"
+    try:
+        with pytest.raises(jb.JBError) as ei:
+            jb.GroupedPlan("f64", descs)
+        assert ei.value.code == 4
+    finally:
+        del os.environ["JB_FAULT_GRAPH"]
+    os.environ["JB_FAULT_ALLOC"] = "1"
+    try:
+        with pytest.raises(jb.JBError) as ei:
+            jb.gemm_grouped_("f64", descs)
+        assert ei.value.code == 2
+    finally:
+        del os.environ["JB_FAULT_ALLOC"]
+    plan = jb.GroupedPlan("f64", descs)
+    plan.launch()
+    torch.cuda.synchronize()
+    plan.destroy()
+    with pytest.raises(ValueError):
+        jb.gemm_grouped_("f64", [GemmDesc("N", "N", 1.0, probs[0][5], probs[1][6], 0.0, probs[0][7])])
+
+
+# ---------------------------------------------------------------------------
+# rrule / pullback
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("method", ["f32", "f32_bf16x3"])
+def test_pullback_c5(gpu, method):
+    """Config: F32 A 4096x2048, B 2048x4096 ~ N(0,1)/sqrt(2048), W ~ N(0,1);
+    dA = W*B^T, dB = A^T*W checked on sampled rows/columns against the FP64 oracle."""
+    torch = gpu
+    M, K, N = 4096, 2048, 4096
+    A = (torch.from_numpy(G.normal(42, K * M).reshape(K, M)) / np.sqrt(2048)).float().cuda().T
+    B = (torch.from_numpy(G.normal(43, N * K).reshape(N, K)) / np.sqrt(2048)).float().cuda().T
+    W = torch.from_numpy(G.normal(45, N * M).reshape(N, M)).float().cuda().T
+    dA = jb.device_matrix(M, K, torch.float32)
+    dB = jb.device_matrix(K, N, torch.float32)
+    from jblas_amd.rrule import pullback_
+
+    pullback_("N", "N", 1.0, A, B, W, dA, dB, method=method)
+    torch.cuda.synchronize()
+    rows = np.random.default_rng(0).choice(M, 24, replace=False)
+    cols = np.random.default_rng(1).choice(N, 24, replace=False)
+    Ah, Bh, Wh = to_host(A), to_host(B), to_host(W)
+    # dA rows: W[rows, :] * B^T  (k = N)
+    ref_a = O.sgemm_dot2("N", "T", 1.0, Wh[rows, :], Bh, 0.0, np.zeros((24, K), np.float32))
+    abs_a = np.abs(Wh[rows, :]).astype(float) @ np.abs(Bh.T).astype(float)
+    got_a = to_host(dA)[rows, :].astype(float)
+    assert np.max(np.abs(got_a - ref_a) / abs_a) <= 2 * N * U32
+    # dB columns: A^T * W[:, cols]  (k = M)
+    ref_b = O.sgemm_dot2("T", "N", 1.0, Ah, Wh[:, cols], 0.0, np.zeros((K, 24), np.float32))
+    abs_b = np.abs(Ah.T).astype(float) @ np.abs(Wh[:, cols]).astype(float)
+    got_b = to_host(dB)[:, cols].astype(float)
+    assert np.max(np.abs(got_b - ref_b) / abs_b) <= 2 * M * U32
+
+
+def test_rrule_fp64_finite_differences_and_cpu_gradient(gpu):
+    """64^3 sub-problem in FP64: loss = sum(W .* (A*B)); FP64 central differences,
+    h = 1e-6, and the oracle's recurrence for W*B^T / A^T*W (bit-exact)."""
+    torch = gpu
+    n = 64
+    A = G.matrix(42, n, n, kind="normal") / np.sqrt(2048)
+    B = G.matrix(43, n, n, kind="normal") / np.sqrt(2048)
+    W = G.matrix(45, n, n, kind="normal")
+    from jblas_amd.rrule import rrule_gemm
+
+    Y, back = rrule_gemm("N", "N", 1.0, to_dev(torch, A), to_dev(torch, B), 0.0, to_dev(torch, np.zeros((n, n))))
+    dA, dB, dalpha, dbeta, dC = back(to_dev(torch, W))
+    torch.cuda.synchronize()
+    dA, dB = to_host(dA), to_host(dB)
+    assert np.array_equal(dA, O.gemm_recurrence("N", "T", 1.0, W, B, 0.0, np.zeros((n, n))))
+    assert np.array_equal(dB, O.gemm_recurrence("T", "N", 1.0, A, W, 0.0, np.zeros((n, n))))
+
+    def loss(Am, Bm):
+        return float(np.sum(W * (Am @ Bm)))
+
+    h = 1e-6
+    rng = np.random.default_rng(3)
+    L = abs(loss(A, B))
+    for (i, j) in rng.integers(0, n, (40, 2)):
+        E = np.zeros((n, n))
+        E[i, j] = h
+        fd_a = (loss(A + E, B) - loss(A - E, B)) / (2 * h)
+        fd_b = (loss(A, B + E) - loss(A, B - E)) / (2 * h)
+        tol = 1e-7 * max(1.0, L)
+        assert abs(fd_a - dA[i, j]) <= tol and abs(fd_b - dB[i, j]) <= tol
+    Yh = to_host(Y)
+    assert abs(float(dalpha) - np.sum(W * Yh)) <= 1e-12 * np.sum(np.abs(W * Yh))
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_rrule_ops_closed_form(gpu, ta, tb):
+    torch = gpu
+    m, n, k = 50, 37, 29
+    A = G.matrix(1, m if ta == "N" else k, k if ta == "N" else m)
+    B = G.matrix(2, k if tb == "N" else n, n if tb == "N" else k)
+    C = G.matrix(3, m, n)
+    dY = G.matrix(4, m, n)
+    from jblas_amd.rrule import rrule_gemm
+
+    Y, back = rrule_gemm(ta, tb, 1.5, to_dev(torch, A), to_dev(torch, B), 0.5, to_dev(torch, C))
+    dA, dB, dalpha, dbeta, dC = back(to_dev(torch, dY))
+    torch.cuda.synchronize()
+    Ao, Bo = (A if ta == "N" else A.T), (B if tb == "N" else B.T)
+    ref_dAo = 1.5 * dY @ Bo.T
+    ref_dBo = 1.5 * Ao.T @ dY
+    ref_dA = ref_dAo if ta == "N" else ref_dAo.T
+    ref_dB = ref_dBo if tb == "N" else ref_dBo.T
+    assert np.allclose(to_host(dA), ref_dA, rtol=0, atol=1e-12)
+    assert np.allclose(to_host(dB), ref_dB, rtol=0, atol=1e-12)
+    assert np.allclose(to_host(Y), 1.5 * Ao @ Bo + 0.5 * C, rtol=0, atol=1e-12)
+    assert abs(float(dalpha) - np.sum(dY * (Ao @ Bo))) < 1e-9
+    assert abs(float(dbeta) - np.sum(dY * C)) < 1e-9
+    assert np.array_equal(to_host(dC), 0.5 * dY)
+
+
+def test_batched_backward_256(gpu):
+    """Config: batched backward over 256 uneven shapes (C4 distribution), FP32 3xBF16."""
+    torch = gpu
+    from jblas_amd.rrule import pullback_grouped_
+
+    shapes = G.loguniform_shapes(62, 256)
+    items, host = [], []
+    for i, (m, n, k) in enumerate(shapes):
+        A = dev_uniform(torch, int(m), int(k), 5000 + 4 * i, torch.float32)
+        B = dev_uniform(torch, int(k), int(n), 5001 + 4 * i, torch.float32)
+        dY = dev_uniform(torch, int(m), int(n), 5002 + 4 * i, torch.float32)
+        dA = jb.device_matrix(int(m), int(k), torch.float32)
+        dB = jb.device_matrix(int(k), int(n), torch.float32)
+        items.append(("N", "N", 1.0, A, B, dY, dA, dB))
+    pullback_grouped_(items, method="f32_bf16x3")
+    torch.cuda.synchronize()
+    for (ta, tb, al, A, B, dY, dA, dB) in items[::16]:
+        Ah, Bh, dYh = to_host(A), to_host(B), to_host(dY)
+        k = Ah.shape[1]
+        ref_a = O.sgemm_dot2("N", "T", 1.0, dYh, Bh, 0.0, np.zeros(Ah.shape, np.float32))
+        abs_a = np.abs(dYh).astype(float) @ np.abs(Bh.T).astype(float)
+        assert np.max(np.abs(to_host(dA) - ref_a) / abs_a) <= 2 * dYh.shape[1] * U32
+        ref_b = O.sgemm_dot2("T", "N", 1.0, Ah, dYh, 0.0, np.zeros(Bh.shape, np.float32))
+        abs_b = np.abs(Ah.T).astype(float) @ np.abs(dYh).astype(float)
+        assert np.max(np.abs(to_host(dB) - ref_b) / abs_b) <= 2 * Ah.shape[0] * U32
+
+
+def test_torch_autograd_binding(gpu):
+    torch = gpu
+    from jblas_amd.rrule import matmul
+
+    A = torch.randn(70, 40, dtype=torch.float64, device="cuda", requires_grad=True)
+    B = torch.randn(40, 30, dtype=torch.float64, device="cuda", requires_grad=True)
+    W = torch.randn(70, 30, dtype=torch.float64, device="cuda")
+    (matmul(A, B) * W).sum().backward()
+    Ac, Bc = A.detach().cpu().requires_grad_(), B.detach().cpu().requires_grad_()
+    (Ac @ Bc * W.cpu()).sum().backward()
+    assert torch.allclose(A.grad.cpu(), Ac.grad, rtol=0, atol=1e-12)
+    assert torch.allclose(B.grad.cpu(), Bc.grad, rtol=0, atol=1e-12)

@@ -128,6 +128,30 @@ def test_gemm_recurrence_matches_jmul_and_ops():
     assert np.array_equal(s, O.gemm_recurrence("N", "N", 1.5, A, B, 0.5, C0)[ii, jj])
 
 
+@pytest.mark.parametrize("kc", [1, 32, 64, 96, 200, 1000])
+def test_gemm_recurrence_chunked_semantics(kc):
+    # chunk q is the plain recurrence on the k-slice; chunks are summed in order
+    A = G.matrix(13, 37, 200)
+    B = G.matrix(14, 200, 23)
+    C0 = G.matrix(15, 37, 23)
+    got = O.gemm_recurrence_chunked("T", "N", 1.5, np.asfortranarray(A.T), B, 0.5, C0, kc)
+    s = None
+    for c0 in range(0, 200, kc):
+        sl = slice(c0, min(200, c0 + kc))
+        part = O.gemm_recurrence("N", "N", 1.0, np.asfortranarray(A[:, sl]), np.asfortranarray(B[sl, :]), 0.0,
+                                 np.zeros((37, 23)))
+        s = part if s is None else s + part
+    ref = 1.5 * s  # c = alpha*s (one rounding)
+    # then c = fma(beta, c0, c): one rounding of the exact beta*c0 + c
+    fin = np.empty_like(ref)
+    for i in range(37):
+        for j in range(23):
+            fin[i, j] = float(Fraction(0.5) * Fraction(C0[i, j]) + Fraction(ref[i, j]))
+    assert np.array_equal(got, fin)
+    if kc >= 200:
+        assert np.array_equal(got, O.gemm_recurrence("N", "N", 1.5, A, B, 0.5, C0))
+
+
 # ---------------------------------------------------------------------------
 # Dot2 FP64 oracle
 # ---------------------------------------------------------------------------
