@@ -1,0 +1,196 @@
+"""
+    JBlasAMD
+
+Julia front end of libjblas_hip.so, the MI355X (gfx950) GEMM engine behind
+jBLAS.jl's GEMM entry point.  Drop-ins:
+
+  * `jmul!(D, A, X)`  -- D = A*X with jBLAS's per-element fma recurrence
+    (src/gemm.jl:244-348 of jBLAS.jl), bit-identical, fringe included;
+    accepts any strided `Matrix{Float64}` (the reference takes `MMatrix`).
+  * `gemm!(tA, tB, α, A, B, β, C; method)` -- LinearAlgebra.BLAS.gemm!
+    argument order; Float64 methods `:native` (FP64 MFMA), `:recurrence`,
+    `:ozaki` (split-BF16, keyword `nslices`); Float32 `:native`, `:bf16x3`.
+  * `kernel!(D, A, X)` / `initkernel!(D, A, X)` -- D += A*X / D = A*X on
+    strided sub-blocks (src/kernels.jl:212-275).
+  * `gemm(tA, tB, α, A, B)` -- non-mutating form with a ChainRulesCore
+    `rrule` (package extension) backed by the library's pullback entry point.
+  * `GroupedPlan` -- many GEMMs in one persistent, hipGraph-captured launch.
+
+All calls go through `ccall` into the C ABI declared in include/jblas_hip.h.
+Host arrays use the synchronous host-pointer entry points; device arrays
+(e.g. AMDGPU.ROCArray) can be passed to the `*_dev` functions as raw pointers.
+"""
+module JBlasAMD
+
+using LinearAlgebra
+
+export jmul!, gemm!, gemm, kernel!, initkernel!, GroupedPlan, launch!, destroy!
+
+const libjblas = get(ENV, "JB_LIBRARY", normpath(joinpath(@__DIR__, "..", "..", "..", "lib", "libjblas_hip.so")))
+
+const JB_F64_NATIVE = Cint(0)
+const JB_F64_RECURRENCE = Cint(1)
+const JB_F32_NATIVE = Cint(2)
+const JB_F32_BF16X3 = Cint(3)
+
+struct JBError <: Exception
+    func::Symbol
+    code::Cint
+    msg::String
+end
+Base.showerror(io::IO, e::JBError) =
+    print(io, "JBError in ", e.func, ": ", e.code < 0 ? "argument $(-e.code) invalid" : "code $(e.code)", ": ", e.msg)
+
+last_error() = unsafe_string(ccall((:jb_last_error, libjblas), Cstring, ()))
+check(func::Symbol, rc::Cint) = rc == 0 ? nothing : throw(JBError(func, rc, last_error()))
+
+colmajor(A::StridedMatrix) = stride(A, 1) == 1
+ld(A::StridedMatrix) = max(1, stride(A, 2), size(A, 1))
+
+function jmul!(D::StridedMatrix{Float64}, A::StridedMatrix{Float64}, X::StridedMatrix{Float64})
+    M, N = size(A)
+    N2, P = size(X)
+    (N == N2 && size(D) == (M, P)) || throw(DimensionMismatch("D $(size(D)), A $(size(A)), X $(size(X))"))
+    if colmajor(D) && colmajor(A) && colmajor(X) && ld(D) == max(1, M) && ld(A) == max(1, M) && ld(X) == max(1, N)
+        check(:jb_jmul, ccall((:jb_jmul, libjblas), Cint,
+                              (Int64, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}), M, N, P, D, A, X))
+        return D
+    end
+    gemm!('N', 'N', 1.0, A, X, 0.0, D; method = :recurrence)
+end
+
+function _dims(tA, tB, A, B, C)
+    m, k = tA == 'N' ? size(A) : reverse(size(A))
+    k2, n = tB == 'N' ? size(B) : reverse(size(B))
+    (k == k2 && size(C) == (m, n)) || throw(DimensionMismatch("op(A) $m×$k, op(B) $k2×$n, C $(size(C))"))
+    m, n, k
+end
+
+function gemm!(tA::AbstractChar, tB::AbstractChar, α::Real, A::StridedMatrix{Float64}, B::StridedMatrix{Float64},
+               β::Real, C::StridedMatrix{Float64}; method::Symbol = :native, nslices::Integer = 7)
+    (colmajor(A) && colmajor(B) && colmajor(C)) || throw(ArgumentError("unit-stride columns required"))
+    m, n, k = _dims(tA, tB, A, B, C)
+    if method === :native
+        check(:jb_dgemm, ccall((:jb_dgemm, libjblas), Cint,
+            (Cchar, Cchar, Int64, Int64, Int64, Float64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Float64,
+             Ptr{Float64}, Int64), tA, tB, m, n, k, α, A, ld(A), B, ld(B), β, C, ld(C)))
+    elseif method === :recurrence
+        check(:jb_dgemm_recurrence, ccall((:jb_dgemm_recurrence, libjblas), Cint,
+            (Cchar, Cchar, Int64, Int64, Int64, Float64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Float64,
+             Ptr{Float64}, Int64), tA, tB, m, n, k, α, A, ld(A), B, ld(B), β, C, ld(C)))
+    elseif method === :ozaki
+        check(:jb_dgemm_ozaki, ccall((:jb_dgemm_ozaki, libjblas), Cint,
+            (Cchar, Cchar, Int64, Int64, Int64, Float64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Float64,
+             Ptr{Float64}, Int64, Cint), tA, tB, m, n, k, α, A, ld(A), B, ld(B), β, C, ld(C), nslices))
+    else
+        throw(ArgumentError("Float64 method must be :native, :recurrence or :ozaki"))
+    end
+    C
+end
+
+function gemm!(tA::AbstractChar, tB::AbstractChar, α::Real, A::StridedMatrix{Float32}, B::StridedMatrix{Float32},
+               β::Real, C::StridedMatrix{Float32}; method::Symbol = :native, nprod::Integer = 6)
+    (colmajor(A) && colmajor(B) && colmajor(C)) || throw(ArgumentError("unit-stride columns required"))
+    m, n, k = _dims(tA, tB, A, B, C)
+    if method === :native
+        check(:jb_sgemm, ccall((:jb_sgemm, libjblas), Cint,
+            (Cchar, Cchar, Int64, Int64, Int64, Float32, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Float32,
+             Ptr{Float32}, Int64), tA, tB, m, n, k, α, A, ld(A), B, ld(B), β, C, ld(C)))
+    elseif method === :bf16x3
+        check(:jb_sgemm_bf16x3, ccall((:jb_sgemm_bf16x3, libjblas), Cint,
+            (Cchar, Cchar, Int64, Int64, Int64, Float32, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Float32,
+             Ptr{Float32}, Int64, Cint), tA, tB, m, n, k, α, A, ld(A), B, ld(B), β, C, ld(C), nprod))
+    else
+        throw(ArgumentError("Float32 method must be :native or :bf16x3"))
+    end
+    C
+end
+
+"D += A*X on strided sub-blocks (kernel!, src/kernels.jl:212-241)."
+kernel!(D, A, X) = gemm!('N', 'N', 1.0, A, X, 1.0, D; method = :recurrence)
+"D = A*X on strided sub-blocks (initkernel!, src/kernels.jl:242-275)."
+initkernel!(D, A, X) = gemm!('N', 'N', 1.0, A, X, 0.0, D; method = :recurrence)
+
+"Non-mutating α*op(A)*op(B); differentiable through the ChainRulesCore extension."
+function gemm(tA::AbstractChar, tB::AbstractChar, α::Real, A::StridedMatrix{T}, B::StridedMatrix{T};
+              kw...) where {T<:Union{Float32,Float64}}
+    m = tA == 'N' ? size(A, 1) : size(A, 2)
+    n = tB == 'N' ? size(B, 2) : size(B, 1)
+    gemm!(tA, tB, α, A, B, zero(T), Matrix{T}(undef, m, n); kw...)
+end
+
+"""
+    pullback!(tA, tB, α, A, B, dC, dA, dB; method)
+
+dA = α dC op(B)ᵀ (or its transpose for tA = 'T') and dB = α op(A)ᵀ dC (or
+transposed for tB = 'T'), both in one grouped launch (jb_gemm_pullback).
+Either of dA, dB may be `nothing`.
+"""
+function pullback!(tA::AbstractChar, tB::AbstractChar, α::Real, A::StridedMatrix{T}, B::StridedMatrix{T},
+                   dC::StridedMatrix{T}, dA, dB; method::Symbol = :native,
+                   kw...) where {T<:Union{Float32,Float64}}
+    m, k = tA == 'N' ? size(A) : reverse(size(A))
+    n = tB == 'N' ? size(B, 2) : size(B, 1)
+    size(dC) == (m, n) || throw(DimensionMismatch("dC must be $m×$n"))
+    mid = T === Float64 ? JB_F64_NATIVE : (method === :bf16x3 ? JB_F32_BF16X3 : JB_F32_NATIVE)
+    pA = dA === nothing ? Ptr{T}(C_NULL) : pointer(dA)
+    pB = dB === nothing ? Ptr{T}(C_NULL) : pointer(dB)
+    GC.@preserve dA dB begin
+        check(:jb_gemm_pullback, ccall((:jb_gemm_pullback, libjblas), Cint,
+            (Cint, Cchar, Cchar, Int64, Int64, Int64, Float64, Ptr{T}, Int64, Ptr{T}, Int64, Ptr{T}, Int64,
+             Ptr{T}, Int64, Ptr{T}, Int64),
+            mid, tA, tB, m, n, k, α, A, ld(A), B, ld(B), dC, ld(dC),
+            pA, dA === nothing ? 1 : ld(dA), pB, dB === nothing ? 1 : ld(dB)))
+    end
+    dA, dB
+end
+
+# ---- grouped launches on device memory ---------------------------------------
+"C layout of jb_gemm_desc (include/jblas_hip.h)."
+struct GemmDesc
+    transA::Cchar
+    transB::Cchar
+    pad::NTuple{6,UInt8}
+    m::Int64
+    n::Int64
+    k::Int64
+    alpha::Float64
+    beta::Float64
+    A::Ptr{Cvoid}
+    lda::Int64
+    B::Ptr{Cvoid}
+    ldb::Int64
+    C::Ptr{Cvoid}
+    ldc::Int64
+end
+GemmDesc(tA, tB, m, n, k, α, A, lda, B, ldb, β, C, ldc) =
+    GemmDesc(Cchar(tA), Cchar(tB), ntuple(_ -> 0x00, 6), m, n, k, α, β, A, lda, B, ldb, C, ldc)
+
+"A hipGraph-captured persistent launch of many GEMMs on device pointers."
+mutable struct GroupedPlan
+    handle::Ptr{Cvoid}
+    descs::Vector{GemmDesc}
+end
+
+function GroupedPlan(method::Cint, descs::Vector{GemmDesc}; stream::Ptr{Cvoid} = C_NULL)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(:jb_grouped_plan_create, ccall((:jb_grouped_plan_create, libjblas), Cint,
+        (Cint, Int64, Ptr{GemmDesc}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}), method, length(descs), descs, stream, h))
+    p = GroupedPlan(h[], descs)
+    finalizer(destroy!, p)
+    p
+end
+
+launch!(p::GroupedPlan; stream::Ptr{Cvoid} = C_NULL) =
+    check(:jb_grouped_plan_launch, ccall((:jb_grouped_plan_launch, libjblas), Cint, (Ptr{Cvoid}, Ptr{Cvoid}),
+                                         p.handle, stream))
+
+function destroy!(p::GroupedPlan)
+    if p.handle != C_NULL
+        ccall((:jb_grouped_plan_destroy, libjblas), Cint, (Ptr{Cvoid},), p.handle)
+        p.handle = C_NULL
+    end
+    nothing
+end
+
+end # module

@@ -31,6 +31,23 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.SIGNATURES) == syms
 
 
+def test_julia_binding_names_declared_symbols():
+    """The Julia wrapper (shipped as source; no Julia in this image) only
+    ccalls symbols the header declares, with the header's argument count."""
+    src = os.path.join(ROOT, "jblas.jl_amd", "julia", "JBlasAMD", "src", "JBlasAMD.jl")
+    text = open(src).read()
+    called = re.findall(r"ccall\(\(:(jb_[a-z0-9_]+), libjblas\), \w+,\s*\(([^)]*)\)", text)
+    assert len(called) >= 10
+    header = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    for name, argtypes in called:
+        m = re.search(r"\b" + name + r"\s*\(([^)]*)\)", header)
+        assert m, name
+        n_header = 0 if m.group(1).strip() in ("", "void") else m.group(1).count(",") + 1
+        argtypes = argtypes.strip().rstrip(",")  # Julia 1-tuples carry a trailing comma
+        n_julia = 0 if not argtypes else argtypes.count(",") + 1
+        assert n_header == n_julia, (name, n_header, n_julia)
+
+
 def test_version_and_error_string():
     assert _lib.lib().jb_version() >= 10000
     assert isinstance(_lib.last_error(), str)
