@@ -48,6 +48,9 @@ constexpr int QPAD_K = 64;          // k padding of the planes (split tile width
 #ifndef OZ_WB
 #define OZ_WB 1  // emulated FP64: 8 waves of 32 x 32 (the FP64 group accumulators need the registers)
 #endif
+#ifndef OZ_GROUP
+#define OZ_GROUP 2  // tile-rows per group of the tile order (measured best of 2, 4, 8, 16)
+#endif
 #ifndef X3_WB
 #define X3_WB 2  // 3xBF16: 4 waves of 64 x 32
 #endif
@@ -271,14 +274,22 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   const int nblocks = p.tiles_m * p.tiles_n;
   const int tile = xcd_remap(blockIdx.x, nblocks, 8);
   int tm, tn;
-  tile_coords(tile, p.tiles_m, p.tiles_n, 8, tm, tn);
+  tile_coords(tile, p.tiles_m, p.tiles_n, OZ_GROUP, tm, tn);
   const int64_t m0 = (int64_t)tm * QBM, n0 = (int64_t)tn * QBN;
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const int wm0 = 32 * WB * (wave % WM), wn0 = 32 * (wave / WM);
   const int l32 = lane % 32, hl = lane / 32;
+  const int phl = hl ^ ((l32 >> 3) & 1);  // physical half of this lane's fragment in the LDS image
 
-  const uint16_t* gA = p.pa + (int64_t)tm * QBM * p.Kpad + lane * 8;
-  const uint16_t* gB = p.pb + (int64_t)tn * QBN * p.Kpad + lane * 8;
+  // LDS image of a plane stage: 32 bytes per row (two 16-byte halves of 8 k),
+  // with the halves of rows whose bit 3 is set swapped, so that the 16 lanes
+  // of one ds_read_b128 phase (16 rows, same half) hit 16 distinct 4-bank
+  // groups of the 64-bank LDS.  A DMA lane writes LDS chunk `lane` of its
+  // 1 KB piece (row lane/2, physical half lane%2) and so fetches the logical
+  // half (lane%2) ^ bit 3 of its row from global memory.
+  const int dma_off = (lane / 2) * QBK + 8 * ((lane % 2) ^ ((lane >> 4) & 1));
+  const uint16_t* gA = p.pa + (int64_t)tm * QBM * p.Kpad + dma_off;
+  const uint16_t* gB = p.pb + (int64_t)tn * QBN * p.Kpad + dma_off;
   auto issue = [&](int ks, int slot) {
     char* base = ring + slot * STAGE_B;
     for (int j = wave; j < NWL; j += NWAVES) {
@@ -326,13 +337,13 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
     const bool chunk_end = (MODE == 0) && ((((st + 1) % KC_ST) == 0) || (st + 1 == nst));
     bf8 bfr[S];
 #pragma unroll
-    for (int u = 0; u < S; ++u) bfr[u] = __builtin_bit_cast(bf8, L[u * (PLANE_B / 16) + (QBM + wn0 + l32) * 2 + hl]);
+    for (int u = 0; u < S; ++u) bfr[u] = __builtin_bit_cast(bf8, L[u * (PLANE_B / 16) + (QBM + wn0 + l32) * 2 + phl]);
 #pragma unroll
     for (int t = 0; t < S; ++t) {
       bf8 a[WB];
 #pragma unroll
       for (int b = 0; b < WB; ++b)
-        a[b] = __builtin_bit_cast(bf8, L[t * (PLANE_B / 16) + (wm0 + 32 * b + l32) * 2 + hl]);
+        a[b] = __builtin_bit_cast(bf8, L[t * (PLANE_B / 16) + (wm0 + 32 * b + l32) * 2 + phl]);
 #pragma unroll
       for (int u = 0; u < S; ++u) {
         if (t + u + 2 > LMAX) continue;
