@@ -107,6 +107,23 @@ int jb_jmul(int64_t M, int64_t N, int64_t P, double* D, const double* A, const d
 int jb_jmul_dev(int64_t M, int64_t N, int64_t P, double* D, const double* A, const double* X,
                 void* stream);
 
+/* ---- fastmul! drop-in ---------------------------------------------------------- */
+/* D (M x P) = A (M x N) * X (N x P), column-major, leading dimensions M, N, M, as
+ * one small-GEMM launch: one wave per 64 rows x 16 columns of D, X read through
+ * wave-uniform (scalar) loads.  Same per-element recurrence as jb_jmul
+ * (mulinit's plain product src/kernels.jl:3-21, then the vmuladd k-loop :93-99),
+ * so the bits equal jb_jmul's.  Meant for small matrices (latency, not
+ * throughput); any size is accepted.  Replaces fastmul!(D, A, X)
+ * src/kernels.jl:202-208 (which, for M not a multiple of its vector width,
+ * stores past D at :110 and, with N == 1, adds the single product twice; here
+ * rows >= M are not stored and N == 1 gives D = A*X). */
+int jb_fastmul(int64_t M, int64_t N, int64_t P, double* D, const double* A, const double* X);
+int jb_fastmul_dev(int64_t M, int64_t N, int64_t P, double* D, const double* A, const double* X,
+                   void* stream);
+int jb_fastmul_f32(int64_t M, int64_t N, int64_t P, float* D, const float* A, const float* X);
+int jb_fastmul_f32_dev(int64_t M, int64_t N, int64_t P, float* D, const float* A, const float* X,
+                       void* stream);
+
 /* ---- BLAS GEMM (gemm! in Julia: LinearAlgebra.BLAS.gemm!(tA,tB,alpha,A,B,beta,C)) */
 /* FP64 on FP64 MFMA.  jmul!(D,A,X) == jb_dgemm('N','N',M,P,N,1,A,M,X,N,0,D,M)
  * up to the summation order of the matrix core. */

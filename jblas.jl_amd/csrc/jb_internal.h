@@ -27,6 +27,11 @@ hipError_t launch_dgemm_recurrence(const GemmArgs& g, hipStream_t s);
 // C = beta*C (alpha == 0 or k == 0 quick path); beta == 0 writes zeros.
 hipError_t launch_scale_f64(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipStream_t s);
 hipError_t launch_scale_f32(int64_t m, int64_t n, float beta, float* C, int64_t ldc, hipStream_t s);
+// Whole-matrix small GEMM D = A*X (fastmul!), one fma chain per element.
+hipError_t launch_fastmul_f64(int64_t M, int64_t N, int64_t P, double* D, const double* A, const double* X,
+                              hipStream_t s);
+hipError_t launch_fastmul_f32(int64_t M, int64_t N, int64_t P, float* D, const float* A, const float* X,
+                              hipStream_t s);
 
 // ---- split-BF16 emulated FP64 --------------------------------------------
 // Workspace bytes needed for an m x n x k problem with s slices.

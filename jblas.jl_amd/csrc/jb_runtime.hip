@@ -435,6 +435,58 @@ int jb_jmul(int64_t M, int64_t N, int64_t P, double* D, const double* A, const d
   return jb_dgemm_recurrence('N', 'N', M, P, N, 1.0, A, M > 1 ? M : 1, X, N, 0.0, D, M > 1 ? M : 1);
 }
 
+// ---- fastmul! (whole-matrix small GEMM) -----------------------------------
+extern "C++" {
+template <typename T>
+static int fastmul_dev_locked(int64_t M, int64_t N, int64_t P, T* D, const T* A, const T* X, hipStream_t s) {
+  hipError_t e = sizeof(T) == 8
+                     ? launch_fastmul_f64(M, N, P, (double*)D, (const double*)A, (const double*)X, s)
+                     : launch_fastmul_f32(M, N, P, (float*)D, (const float*)A, (const float*)X, s);
+  return e == hipSuccess ? JB_OK : hip_fail(e, "fastmul launch");
+}
+
+template <typename T>
+static int fastmul_entry(int64_t M, int64_t N, int64_t P, T* D, const T* A, const T* X, void* stream, bool host) {
+  int rc = jmul_validate(M, N, P, D, A, X);
+  if (rc) return rc;
+  // grid: x = row blocks of 64 (<= 2^24), y = column strips of 16 (<= 65535)
+  if (M > (int64_t(1) << 30) || P > int64_t(16) * 65535)
+    return fail(JB_ERR_UNSUPPORTED, "fastmul is a small-matrix path: M <= 2^30 and P <= 1048560");
+  std::lock_guard<std::mutex> lk(rt().mu);
+  if (M == 0 || P == 0) return JB_OK;
+  rc = ensure_init();
+  if (rc) return rc;
+  if (!host) return fastmul_dev_locked<T>(M, N, P, D, A, X, (hipStream_t)stream);
+  GemmArgs g{};
+  g.m = M;
+  g.n = P;
+  g.k = N;
+  g.alpha = 1.0;
+  g.A = A;
+  g.lda = M;
+  g.B = X;
+  g.ldb = N;
+  g.C = D;
+  g.ldc = M;
+  return run_staged(g, sizeof(T), false, [](const GemmArgs& d, hipStream_t s) {
+    return fastmul_dev_locked<T>(d.m, d.k, d.n, (T*)d.C, (const T*)d.A, (const T*)d.B, s);
+  });
+}
+}  // extern "C++"
+
+int jb_fastmul(int64_t M, int64_t N, int64_t P, double* D, const double* A, const double* X) {
+  return fastmul_entry<double>(M, N, P, D, A, X, nullptr, true);
+}
+int jb_fastmul_dev(int64_t M, int64_t N, int64_t P, double* D, const double* A, const double* X, void* stream) {
+  return fastmul_entry<double>(M, N, P, D, A, X, stream, false);
+}
+int jb_fastmul_f32(int64_t M, int64_t N, int64_t P, float* D, const float* A, const float* X) {
+  return fastmul_entry<float>(M, N, P, D, A, X, nullptr, true);
+}
+int jb_fastmul_f32_dev(int64_t M, int64_t N, int64_t P, float* D, const float* A, const float* X, void* stream) {
+  return fastmul_entry<float>(M, N, P, D, A, X, stream, false);
+}
+
 // ---- FP64 emulated (split-BF16) -------------------------------------------
 static int ozaki_dev_locked(const GemmArgs& g, int s, hipStream_t st) {
   if (g.m == 0 || g.n == 0) return JB_OK;

@@ -276,7 +276,9 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   int tm, tn;
   tile_coords(tile, p.tiles_m, p.tiles_n, OZ_GROUP, tm, tn);
   const int64_t m0 = (int64_t)tm * QBM, n0 = (int64_t)tn * QBN;
-  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int tid = threadIdx.x, lane = tid % 64;
+  // wave-uniform (SGPR) so that the DMA issue below is scalar code
+  const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
   const int wm0 = 32 * WB * (wave % WM), wn0 = 32 * (wave / WM);
   const int l32 = lane % 32, hl = lane / 32;
   const int phl = hl ^ ((l32 >> 3) & 1);  // physical half of this lane's fragment in the LDS image
@@ -292,7 +294,10 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   const uint16_t* gB = p.pb + (int64_t)tn * QBN * p.Kpad + dma_off;
   auto issue = [&](int ks, int slot) {
     char* base = ring + slot * STAGE_B;
-    for (int j = wave; j < NWL; j += NWAVES) {
+#pragma unroll
+    for (int i = 0; i < (NWL + NWAVES - 1) / NWAVES; ++i) {
+      const int j = wave + NWAVES * i;  // uniform
+      if (j >= NWL) break;
       const uint16_t* src;
       int loff;
       if (j < 4 * S) {
@@ -338,12 +343,18 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
     bf8 bfr[S];
 #pragma unroll
     for (int u = 0; u < S; ++u) bfr[u] = __builtin_bit_cast(bf8, L[u * (PLANE_B / 16) + (QBM + wn0 + l32) * 2 + phl]);
+    // A fragments are read one t ahead so that their LDS latency hides
+    // behind the previous t's MFMAs
+    bf8 a[WB], a_next[WB];
+#pragma unroll
+    for (int b = 0; b < WB; ++b) a[b] = __builtin_bit_cast(bf8, L[(wm0 + 32 * b + l32) * 2 + phl]);
 #pragma unroll
     for (int t = 0; t < S; ++t) {
-      bf8 a[WB];
+      if (t + 1 < S) {
 #pragma unroll
-      for (int b = 0; b < WB; ++b)
-        a[b] = __builtin_bit_cast(bf8, L[t * (PLANE_B / 16) + (wm0 + 32 * b + l32) * 2 + phl]);
+        for (int b = 0; b < WB; ++b)
+          a_next[b] = __builtin_bit_cast(bf8, L[(t + 1) * (PLANE_B / 16) + (wm0 + 32 * b + l32) * 2 + phl]);
+      }
 #pragma unroll
       for (int u = 0; u < S; ++u) {
         if (t + u + 2 > LMAX) continue;
@@ -364,6 +375,10 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
             G[t / 4][b][e] = fma((double)acc[t][b][e], kScale[t % 4], G[t / 4][b][e]);
             acc[t][b][e] = 0.0f;
           }
+      }
+      if (t + 1 < S) {
+#pragma unroll
+        for (int b = 0; b < WB; ++b) a[b] = a_next[b];
       }
     }
   }

@@ -166,6 +166,28 @@ def jmul_(D, A, X, *, stream=None):
     return D
 
 
+def fastmul_(D, A, X, *, stream=None):
+    """D = A*X as one small-GEMM launch (fastmul!, src/kernels.jl:202-208):
+    the same per-element recurrence as jmul_, Float64 or Float32, dense
+    column-major operands (leading dimensions M, N, M)."""
+    d, a, x = _Operand(D, True, "D"), _Operand(A, False, "A"), _Operand(X, False, "X")
+    M, N = a.rows, a.cols
+    if x.rows != N or d.rows != M or d.cols != x.cols:
+        raise DimensionMismatch(f"D is {d.rows}x{d.cols}, A is {M}x{N}, X is {x.rows}x{x.cols}")
+    if not (d.dtype == a.dtype == x.dtype) or d.dtype not in (np.float64, np.float32):
+        raise TypeError("fastmul! takes three Float64 or three Float32 matrices")
+    P = x.cols
+    if d.ld != max(1, M) or a.ld != max(1, M) or x.ld != max(1, N):
+        raise ValueError("fastmul! takes dense column-major matrices (leading dimensions M, N, M)")
+    f64 = d.dtype == np.float64
+    if d.device:
+        _lib.call("jb_fastmul_dev" if f64 else "jb_fastmul_f32_dev", M, N, P, d.ptr, a.ptr, x.ptr,
+                  _stream_of(True, stream))
+    else:
+        _lib.call("jb_fastmul" if f64 else "jb_fastmul_f32", M, N, P, d.ptr, a.ptr, x.ptr)
+    return D
+
+
 def kernel_(D, A, X, *, stream=None):
     """D += A*X on (sub)matrices with their own leading dimensions
     (kernel!, src/kernels.jl:212-241): gemm! with alpha = beta = 1."""

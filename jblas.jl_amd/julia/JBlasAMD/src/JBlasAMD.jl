@@ -10,6 +10,8 @@ jBLAS.jl's GEMM entry point.  Drop-ins:
   * `gemm!(tA, tB, α, A, B, β, C; method)` -- LinearAlgebra.BLAS.gemm!
     argument order; Float64 methods `:native` (FP64 MFMA), `:recurrence`,
     `:ozaki` (split-BF16, keyword `nslices`); Float32 `:native`, `:bf16x3`.
+  * `fastmul!(D, A, X)` -- the same recurrence as one small-GEMM launch
+    (src/kernels.jl:202-208), Float64 or Float32, dense matrices.
   * `kernel!(D, A, X)` / `initkernel!(D, A, X)` -- D += A*X / D = A*X on
     strided sub-blocks (src/kernels.jl:212-275).
   * `gemm(tA, tB, α, A, B)` -- non-mutating form with a ChainRulesCore
@@ -24,7 +26,7 @@ module JBlasAMD
 
 using LinearAlgebra
 
-export jmul!, gemm!, gemm, kernel!, initkernel!, GroupedPlan, launch!, destroy!
+export jmul!, fastmul!, gemm!, gemm, kernel!, initkernel!, GroupedPlan, launch!, destroy!
 
 const libjblas = get(ENV, "JB_LIBRARY", normpath(joinpath(@__DIR__, "..", "..", "..", "lib", "libjblas_hip.so")))
 
@@ -57,6 +59,29 @@ function jmul!(D::StridedMatrix{Float64}, A::StridedMatrix{Float64}, X::StridedM
         return D
     end
     gemm!('N', 'N', 1.0, A, X, 0.0, D; method = :recurrence)
+end
+
+function _fastmul_dims(D, A, X)
+    M, N = size(A)
+    N2, P = size(X)
+    (N == N2 && size(D) == (M, P)) || throw(DimensionMismatch("D $(size(D)), A $(size(A)), X $(size(X))"))
+    (colmajor(D) && colmajor(A) && colmajor(X) && ld(D) == max(1, M) && ld(A) == max(1, M) &&
+     ld(X) == max(1, N)) || throw(ArgumentError("fastmul! takes dense column-major matrices"))
+    M, N, P
+end
+
+function fastmul!(D::StridedMatrix{Float64}, A::StridedMatrix{Float64}, X::StridedMatrix{Float64})
+    M, N, P = _fastmul_dims(D, A, X)
+    check(:jb_fastmul, ccall((:jb_fastmul, libjblas), Cint,
+                             (Int64, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}), M, N, P, D, A, X))
+    D
+end
+
+function fastmul!(D::StridedMatrix{Float32}, A::StridedMatrix{Float32}, X::StridedMatrix{Float32})
+    M, N, P = _fastmul_dims(D, A, X)
+    check(:jb_fastmul_f32, ccall((:jb_fastmul_f32, libjblas), Cint,
+                                 (Int64, Int64, Int64, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}), M, N, P, D, A, X))
+    D
 end
 
 function _dims(tA, tB, A, B, C)

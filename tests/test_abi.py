@@ -83,6 +83,9 @@ def test_other_entry_points_validate():
     assert L.jb_sgemm(b"N", b"Z", 4, 4, 4, 1.0, 1, 4, 1, 4, 0.0, 1, 4) == -2
     assert L.jb_jmul(4, 0, 4, 1, 1, 1) == -2
     assert L.jb_jmul(-1, 2, 4, 1, 1, 1) == -1
+    assert L.jb_fastmul(4, 0, 4, 1, 1, 1) == -2
+    assert L.jb_fastmul_f32(4, 3, -4, 1, 1, 1) == -3
+    assert L.jb_fastmul(4, 3, 4, None, 1, 1) == -4
     assert L.jb_fill_uniform_dev(1, -5, None, None) == -2
     assert L.jb_fill_wide_dev(1, 5, 3, 2, 1, None) == -4
 

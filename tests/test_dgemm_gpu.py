@@ -96,6 +96,40 @@ def test_jmul_sign_of_zero(gpu):
     assert D[0, 0] == 0.0 and math.copysign(1.0, D[0, 0]) < 0  # acc_1 = fl(a*x), src/gemm.jl:86
 
 
+@pytest.mark.parametrize("M,N,P", [(1, 1, 1), (7, 1, 9), (13, 12, 12), (40, 256, 5), (64, 64, 64),
+                                   (65, 33, 17), (200, 5, 300), (3, 1000, 5), (256, 256, 256)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_fastmul_bitexact(gpu, M, N, P, dtype):
+    """fastmul! (src/kernels.jl:202-208): one launch, the jmul! recurrence per
+    element -- bit-identical to the oracle, host and device entry points.
+    N == 1 and M not a multiple of 64 / P not a multiple of 16 are included."""
+    A = G.matrix(31, M, N, dtype=dtype)
+    X = G.matrix(32, N, P, dtype=dtype)
+    ref = O.jmul(A, X)
+    D = np.full((M, P), np.nan, dtype=dtype, order="F")
+    jb.fastmul_(D, A, X)
+    assert np.array_equal(D, ref)
+    Dd = to_dev(gpu, np.full((M, P), np.nan, dtype=dtype))
+    jb.fastmul_(Dd, to_dev(gpu, A), to_dev(gpu, X))
+    gpu.cuda.synchronize()
+    assert np.array_equal(to_host(Dd), ref)
+
+
+def test_fastmul_sign_of_zero_and_neighbours(gpu):
+    # acc_1 = fl(a*x) keeps -0; rows past M are never stored (no write beyond D).
+    D = np.ones((1, 1), order="F")
+    jb.fastmul_(D, np.array([[-1.0]]), np.array([[0.0]]))
+    assert D[0, 0] == 0.0 and math.copysign(1.0, D[0, 0]) < 0
+    M, N, P = 5, 3, 2
+    buf = gpu.full((M * P + 64,), 7.0, dtype=gpu.float64, device="cuda")
+    Dd = buf[: M * P].view(P, M).T  # column-major M x P at the start of a larger buffer
+    A, X = G.matrix(33, M, N), G.matrix(34, N, P)
+    jb.fastmul_(Dd, to_dev(gpu, A), to_dev(gpu, X))
+    gpu.cuda.synchronize()
+    assert np.array_equal(to_host(Dd), O.jmul(A, X))
+    assert np.all(buf[M * P:].cpu().numpy() == 7.0)
+
+
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
 def test_dgemm_c2_1024(gpu, ta, tb):
     """Config: C = 1.5*op(A)*op(B) + 0.5*C, M=N=K=1024, uniform[-1,1) seeds 42/43/44."""

@@ -106,6 +106,41 @@ def test_jmul_float32_restatement():
     assert np.max(np.abs(D - ref)) <= 20 * 2.0 ** -24 * np.max(np.abs(A) @ np.abs(X))
 
 
+def _round_f32(x: Fraction) -> float:
+    """Correct round-to-nearest-even of an exact rational to binary32."""
+    if x == 0:
+        return 0.0
+    sign, x = (-1.0 if x < 0 else 1.0), abs(x)
+    e = x.numerator.bit_length() - x.denominator.bit_length()
+    if Fraction(2) ** e > x:
+        e -= 1
+    q = Fraction(2) ** (max(e, -126) - 23)
+    n = x / q
+    fl = n.numerator // n.denominator
+    r = n - fl
+    if r > Fraction(1, 2) or (r == Fraction(1, 2) and fl % 2 == 1):
+        fl += 1
+    return sign * float(fl * q)
+
+
+def test_jmul_float32_bruteforce():
+    # Per-element chain evaluated with exact rationals and one binary32 rounding
+    # per step: acc = rd(a0*x0), acc = rd(a_k*x_k + acc) (fused, src/gemm.jl:86,165).
+    A = G.matrix(21, 6, 9, dtype=np.float32)
+    X = G.matrix(22, 9, 5, dtype=np.float32)
+    D = O.jmul(A, X)
+    for i in range(6):
+        for j in range(5):
+            acc = _round_f32(Fraction(float(A[i, 0])) * Fraction(float(X[0, j])))
+            for k in range(1, 9):
+                acc = _round_f32(Fraction(float(A[i, k])) * Fraction(float(X[k, j])) + Fraction(acc))
+            assert D[i, j] == np.float32(acc), (i, j)
+    # an unfused multiply-add differs from the chain on this input
+    a = np.float32(1 + 2.0 ** -12)
+    D1 = O.jmul(np.array([[1.0, a]], dtype=np.float32), np.array([[-(1 + 2.0 ** -11)], [a]], dtype=np.float32))
+    assert D1[0, 0] == np.float32(2.0 ** -24)
+
+
 def test_gemm_recurrence_matches_jmul_and_ops():
     A = G.matrix(10, 33, 21)
     B = G.matrix(11, 21, 17)
