@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--skip-table", action="store_true", help="omit the per-slice-count table")
     ap.add_argument("--only", choices=["ozaki", "native"], default=None,
                     help="run only one method (for profiler runs)")
+    ap.add_argument("--skip-grouped", action="store_true",
+                    help="omit the batched (1024 uneven problems) and rrule (4096x2048x4096) measurements")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy a MIB-sized device buffer (a kernel of known traffic for counter scaling)")
     return ap.parse_args()
@@ -59,10 +61,79 @@ def timed(torch, fn, steps):
     return e0.elapsed_time(e1) / steps
 
 
-def cpu_baseline_leg(A_sub_rows, B_sub_cols, A_panel, B_panel, nthreads):
+def grouped_and_rrule(torch, jb):
+    """Secondary workloads, measured after the headline step:
+    * batched: 1024 independent problems, m, n, k log-uniform in [16, 2048]
+      (multiples of 8), one persistent launch captured in a hipGraph, FP64 and
+      FP32 (3xBF16); static tile choice, then after the autotuner;
+    * rrule: F32 A 4096x2048, B 2048x4096: forward + pullback (dA, dB in one
+      grouped launch)."""
+    import numpy as np
+
+    from jblas_amd import generators as G
+    from jblas_amd.grouped import GemmDesc
+    from jblas_amd.rrule import pullback_
+
+    out = {}
+    samples = {}
+    shapes = G.loguniform_shapes(52, 1024)
+    pick = sorted(int(i) for i in np.random.default_rng(53).choice(len(shapes), 8, replace=False))
+    flops = float(sum(2.0 * m * n * k for m, n, k in shapes))
+    for method, dtype in (("f64", torch.float64), ("f32_bf16x3", torch.float32)):
+        descs, keep = [], []
+        for i, (m, n, k) in enumerate(shapes):
+            A = torch.empty((int(k), int(m)), dtype=dtype, device="cuda")
+            B = torch.empty((int(n), int(k)), dtype=dtype, device="cuda")
+            C = torch.zeros((int(n), int(m)), dtype=dtype, device="cuda")
+            jb.fill_uniform_(A, 1000 + 3 * i)
+            jb.fill_uniform_(B, 1001 + 3 * i)
+            descs.append(GemmDesc("N", "N", 1.0, A.T, B.T, 0.0, C.T))
+            keep += [A, B, C]
+        res = {}
+        for phase in ("static", "tuned"):
+            if phase == "tuned":
+                t0 = time.perf_counter()
+                jb.autotune(method, descs, reps=2)
+                res["autotune_seconds"] = round(time.perf_counter() - t0, 2)
+            plan = jb.GroupedPlan(method, descs)
+            ms = timed(torch, plan.launch, 1)  # warm
+            ms = timed(torch, plan.launch, 10)
+            plan.destroy()
+            res[f"{phase}_ms"] = round(ms, 4)
+            res[f"{phase}_tflops"] = round(flops / ms / 1e9, 3)
+        torch.cuda.synchronize()
+        # host copies of a few problems (operands, result) for the per-problem error check
+        samples[method] = [(keep[3 * i].cpu().numpy().T, keep[3 * i + 1].cpu().numpy().T,
+                            keep[3 * i + 2].cpu().numpy().T) for i in pick]
+        out[method] = res
+        del descs, keep
+        torch.cuda.empty_cache()
+    out["batched_config"] = "1024 problems, m,n,k log-uniform in [16,2048] (multiples of 8), seed 52; sum 2mnk = %.4g" % flops
+    # rrule sizes (F32): forward C = A*B and pullback dA = W*B^T, dB = A^T*W
+    M, K, N = 4096, 2048, 4096
+    A = (torch.randn((K, M), dtype=torch.float32, device="cuda") / 45.25).T  # N(0,1)/sqrt(2048), column-major
+    B = (torch.randn((N, K), dtype=torch.float32, device="cuda") / 45.25).T
+    W = torch.randn((N, M), dtype=torch.float32, device="cuda").T
+    Cm = jb.device_matrix(M, N, torch.float32)
+    dA = jb.device_matrix(M, K, torch.float32)
+    dB = jb.device_matrix(K, N, torch.float32)
+    rr = {}
+    for method in ("f32", "f32_bf16x3"):
+        fwd = "native" if method == "f32" else "bf16x3"
+        ms_f = timed(torch, lambda: jb.gemm_("N", "N", 1.0, A, B, 0.0, Cm, method=fwd), 5)
+        ms_b = timed(torch, lambda: pullback_("N", "N", 1.0, A, B, W, dA, dB, method=method), 5)
+        fl = 2.0 * M * N * K
+        rr[method] = {"forward_ms": round(ms_f, 4), "pullback_ms": round(ms_b, 4),
+                      "total_tflops": round(3 * fl / (ms_f + ms_b) / 1e9, 3)}
+    out["rrule_4096x2048x4096_f32"] = rr
+    return out, samples
+
+
+def cpu_baseline_leg(A_sub_rows, B_sub_cols, A_panel, B_panel, nthreads, grouped_samples=None):
     """CPU side, rank 0 only: (1) the oracle's restatement of the reference
     jmul! recurrence timed on a bounded panel of the same workload, (2) the
-    Dot2 FP64 oracle on the sampled rows x columns, used only as the checker."""
+    Dot2 FP64 oracle on the sampled rows x columns and on sampled problems of
+    the batched launch, used only as the checker."""
     import numpy as np
 
     from oracle import oracle as O
@@ -86,6 +157,24 @@ def cpu_baseline_leg(A_sub_rows, B_sub_cols, A_panel, B_panel, nthreads):
     # (2) checker: Dot2 on the sampled block
     out["ref"] = O.gemm_dot2("N", "N", 1.0, A_sub_rows, B_sub_cols, 0.0,
                              np.zeros((A_sub_rows.shape[0], B_sub_cols.shape[1])), nthreads=nthreads)
+    # (3) checker for sampled problems of the batched launch: per-problem error vs Dot2
+    if grouped_samples:
+        errs = {}
+        for method, probs in grouped_samples.items():
+            worst_rel, worst_scaled = 0.0, 0.0
+            for A, B, C in probs:
+                if method == "f64":
+                    ref = O.gemm_dot2("N", "N", 1.0, A, B, 0.0, np.zeros(C.shape), nthreads=nthreads)
+                else:
+                    ref = O.sgemm_dot2("N", "N", 1.0, A, B, 0.0, np.zeros(C.shape, np.float32), nthreads=nthreads)
+                absab = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+                d = np.abs(C.astype(np.float64) - ref)
+                big = np.abs(ref) > 1e-3 * np.max(np.abs(ref))
+                worst_rel = max(worst_rel, float(np.max(d[big] / np.abs(ref[big]))))
+                worst_scaled = max(worst_scaled, float(np.max(d / np.where(absab > 0, absab, 1.0))))
+            errs[method] = {"problems": [list(p[2].shape) + [p[0].shape[1]] for p in probs],
+                            "max_rel_err": worst_rel, "max_err_over_absAB": worst_scaled}
+        out["grouped_errors"] = errs
     return out
 
 
@@ -172,6 +261,12 @@ def main():
                           "out": C.clone() if not args.skip_cpu else None})
             log(f"  table s={ss}: {ms_s:.3f} ms  {flops / ms_s / 1e9:.2f} TF")
 
+    # ---- batched and rrule workloads (after the timed region) ----
+    extra, gsamples = None, None
+    if args.only is None and not args.skip_grouped:
+        extra, gsamples = grouped_and_rrule(torch, jb)
+        log(f"grouped/rrule: {extra}")
+
     # ---- CPU baseline + checker (rank 0, N = 1) ----
     cpu = None
     errs = {}
@@ -187,7 +282,8 @@ def main():
         B_cols = np.asfortranarray(Bh[:, jj])
         panel_cols = int(os.environ.get("JB_CPU_PANEL", "6144"))
         nthreads = max(1, min(16, os.cpu_count() or 1))
-        leg = cpu_baseline_leg(A_rows, B_cols, np.asfortranarray(Ah), np.asfortranarray(Bh[:, :panel_cols]), nthreads)
+        leg = cpu_baseline_leg(A_rows, B_cols, np.asfortranarray(Ah), np.asfortranarray(Bh[:, :panel_cols]), nthreads,
+                               gsamples)
         del Ah, Bh
         ref = leg["ref"]
         absab = np.abs(A_rows) @ np.abs(B_cols)
@@ -205,6 +301,9 @@ def main():
             if "native" in errs:
                 row["err_ratio_vs_native"] = round(row["max_rel_err"] / errs["native"][0], 3)
         cpu = leg["cpu"]
+        if extra is not None and "grouped_errors" in leg:
+            for method, e in leg["grouped_errors"].items():
+                extra[method]["sampled_problem_errors"] = e
         log(f"errors: {errs}; cpu: {cpu}")
     else:
         for row in table:
@@ -276,6 +375,8 @@ def main():
                                    "frac": round(flops / (nat["kernel_ms"] * 1e-3) / 1e12 / info["peak_fp64_tflops"], 4)}
     if table:
         line["nslices_table"] = table
+    if extra:
+        line["batched_and_rrule"] = extra
     if cpu:
         line["cpu_baseline"] = {"value": round(cpu["mt_tflops"], 5), "unit": "TFLOP/s", "cores": cpu["mt_threads"],
                                 "kind": "port",

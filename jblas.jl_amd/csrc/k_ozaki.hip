@@ -330,6 +330,14 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
         for (int e = 0; e < 16; ++e) G[g][b][e] = 0.0;
   }
 
+  // Arguments that only the epilogue reads are pulled into SGPRs here: a scalar
+  // load still in flight across the main loop would turn every LDS wait inside
+  // it into a full lgkmcnt(0) (scalar loads return out of order).
+  int64_t eN = p.N, eldc = p.ldc;
+  double ealpha = p.alpha, ebeta = p.beta;
+  void* eC = p.C;
+  asm volatile("" : "+s"(eN), "+s"(eldc), "+s"(ealpha), "+s"(ebeta), "+s"(eC));
+
   const int nst = (int)(p.Kpad / QBK);
   issue(0, 0);
   if (nst > 1) issue(1, 1);
@@ -340,51 +348,50 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
     if (st + 2 < nst) issue(st + 2, (st + 2) % QSLOTS);
     const u4* L = reinterpret_cast<const u4*>(ring + (st % QSLOTS) * STAGE_B);
     const bool chunk_end = (MODE == 0) && ((((st + 1) % KC_ST) == 0) || (st + 1 == nst));
-    bf8 bfr[S];
+    // Pairs run in square order: step j multiplies the new slices A_j and B_j
+    // with all earlier ones (pairs with max(t, u) == j).  A step needs only two
+    // new fragments, read one step ahead, so a stage starts after two LDS reads
+    // instead of all of them.  Level l = t + u is complete after step l.
+    bf8 af[S][WB], bf[S];
 #pragma unroll
-    for (int u = 0; u < S; ++u) bfr[u] = __builtin_bit_cast(bf8, L[u * (PLANE_B / 16) + (QBM + wn0 + l32) * 2 + phl]);
-    // A fragments are read one t ahead so that their LDS latency hides
-    // behind the previous t's MFMAs
-    bf8 a[WB], a_next[WB];
+    for (int b = 0; b < WB; ++b) af[0][b] = __builtin_bit_cast(bf8, L[(wm0 + 32 * b + l32) * 2 + phl]);
+    bf[0] = __builtin_bit_cast(bf8, L[(QBM + wn0 + l32) * 2 + phl]);
 #pragma unroll
-    for (int b = 0; b < WB; ++b) a[b] = __builtin_bit_cast(bf8, L[(wm0 + 32 * b + l32) * 2 + phl]);
-#pragma unroll
-    for (int t = 0; t < S; ++t) {
-      if (t + 1 < S) {
+    for (int j = 0; j < S; ++j) {
+      if (j + 1 < S) {
 #pragma unroll
         for (int b = 0; b < WB; ++b)
-          a_next[b] = __builtin_bit_cast(bf8, L[(t + 1) * (PLANE_B / 16) + (wm0 + 32 * b + l32) * 2 + phl]);
+          af[j + 1][b] = __builtin_bit_cast(bf8, L[(j + 1) * (PLANE_B / 16) + (wm0 + 32 * b + l32) * 2 + phl]);
+        bf[j + 1] = __builtin_bit_cast(bf8, L[(j + 1) * (PLANE_B / 16) + (QBM + wn0 + l32) * 2 + phl]);
       }
 #pragma unroll
-      for (int u = 0; u < S; ++u) {
+      for (int q = 0; q < 2 * j + 1; ++q) {
+        const int t = q <= j ? j : q - j - 1;  // (j, 0..j), then (0..j-1, j)
+        const int u = q <= j ? q : j;
         if (t + u + 2 > LMAX) continue;
         const int l = (MODE == 0) ? (t + u) : (t + u == 0 ? 0 : 1);
         // X = op(B) plane (p -> n), Y = op(A) plane (q -> m): item r of lane l holds
         // C[m0+wm0+32b+l%32][n0+wn0+8(r/4)+4(l/32)+r%4].
 #pragma unroll
         for (int b = 0; b < WB; ++b)
-          acc[l][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bfr[u], a[b], acc[l][b], 0, 0, 0);
+          acc[l][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bf[u], af[t][b], acc[l][b], 0, 0, 0);
       }
-      if (MODE == 0 && chunk_end) {
-        // level t has received its last product of this chunk: flush it exactly
+      if (MODE == 0 && chunk_end && j < NL) {
+        // level j has received its last product of this chunk: flush it exactly
         constexpr double kScale[4] = {1.0, 0x1.0p-8, 0x1.0p-16, 0x1.0p-24};
 #pragma unroll
         for (int b = 0; b < WB; ++b)
 #pragma unroll
           for (int e = 0; e < 16; ++e) {
-            G[t / 4][b][e] = fma((double)acc[t][b][e], kScale[t % 4], G[t / 4][b][e]);
-            acc[t][b][e] = 0.0f;
+            G[j / 4][b][e] = fma((double)acc[j][b][e], kScale[j % 4], G[j / 4][b][e]);
+            acc[j][b][e] = 0.0f;
           }
-      }
-      if (t + 1 < S) {
-#pragma unroll
-        for (int b = 0; b < WB; ++b) a[b] = a_next[b];
       }
     }
   }
 
   if (MODE == 0) {
-    double* C = static_cast<double*>(p.C);
+    double* C = static_cast<double*>(eC);
 #pragma unroll
     for (int b = 0; b < WB; ++b) {
       const int64_t m = m0 + wm0 + 32 * b + l32;
@@ -393,7 +400,7 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
         const int64_t n = n0 + wn0 + 8 * (e / 4) + 4 * hl + (e % 4);
-        if (n < p.N) {
+        if (n < eN) {
           double hi = 0.0, lo = 0.0;
 #pragma unroll
           for (int g = NG - 1; g >= 0; --g) {
@@ -402,15 +409,15 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
             lo += err;
           }
           const double v = ldexp(hi + lo, ea + p.eB[n] - 14);
-          double c = p.alpha * v;
-          if (p.beta != 0.0) c = fma(p.beta, C[m + n * p.ldc], c);
-          C[m + n * p.ldc] = c;
+          double c = ealpha * v;
+          if (ebeta != 0.0) c = fma(ebeta, C[m + n * eldc], c);
+          C[m + n * eldc] = c;
         }
       }
     }
   } else {
-    float* C = static_cast<float*>(p.C);
-    const float alpha = (float)p.alpha, beta = (float)p.beta;
+    float* C = static_cast<float*>(eC);
+    const float alpha = (float)ealpha, beta = (float)ebeta;
 #pragma unroll
     for (int b = 0; b < WB; ++b) {
       const int64_t m = m0 + wm0 + 32 * b + l32;
@@ -418,10 +425,10 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
         const int64_t n = n0 + wn0 + 8 * (e / 4) + 4 * hl + (e % 4);
-        if (n < p.N) {
+        if (n < eN) {
           float c = alpha * (acc[0][b][e] + acc[1][b][e]);
-          if (beta != 0.0f) c = fmaf(beta, C[m + n * p.ldc], c);
-          C[m + n * p.ldc] = c;
+          if (beta != 0.0f) c = fmaf(beta, C[m + n * eldc], c);
+          C[m + n * eldc] = c;
         }
       }
     }
