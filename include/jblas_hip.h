@@ -203,9 +203,12 @@ int jb_grouped_plan_launch(jb_grouped_plan* plan, void* stream);
 int jb_grouped_plan_destroy(jb_grouped_plan* plan);
 
 /* ---- autotuner ------------------------------------------------------------------ */
-/* Times every candidate (tile shape x split-K) for each size bucket present in
- * `descs` and stores the fastest in the in-memory table (replaces the static
- * pick_kernel_size / blocking_structure choice).  `reps` timed replays each. */
+/* Tunes (tile shape x split-K) per size bucket present in `descs`, in context:
+ * every trial times the whole group (fastest of `reps` launches), buckets are
+ * visited once in decreasing order of their flops (those under 0.5 % of the
+ * total keep the static rule), and a candidate is kept only if it makes the
+ * group >= 1 % faster.  Improved buckets go into the in-memory table (replaces
+ * the static pick_kernel_size / blocking_structure choice). */
 int jb_autotune_grouped(int method, int64_t count, const jb_gemm_desc* descs, int reps,
                         void* stream);
 /* Persist / restore the table as JSON (keyed by device, arch, method, bucket). */
@@ -220,7 +223,9 @@ int jb_autotune_reset(void);
 /* For C = alpha*op(A)*op(B) + beta*C_in and upstream gradient dC (m x n):
  *   dA (shape of A) = alpha * dC * op(B)^T   (transA 'N')  or  alpha * op(B) * dC^T ('T')
  *   dB (shape of B) = alpha * op(A)^T * dC   (transB 'N')  or  alpha * dC^T * op(A) ('T')
- * Both adjoint GEMMs run in one grouped launch (transposed-layout loaders).
+ * Adjoint GEMMs with m*n >= 256 tiles of 128x128 (and k >= 256) run on the
+ * single-problem kernels, one after the other on the stream; smaller ones share
+ * one grouped launch (transposed-layout loaders).
  * dA or dB may be NULL to skip it.  method: JB_F64_NATIVE, JB_F32_NATIVE, JB_F32_BF16X3. */
 int jb_gemm_pullback_dev(int method, char transA, char transB, int64_t m, int64_t n, int64_t k,
                          double alpha, const void* A, int64_t lda, const void* B, int64_t ldb,

@@ -422,9 +422,11 @@ void choose(int method, int64_t m, int64_t n, int64_t k, int& cfg, int& splitk) 
 // ---------------------------------------------------------------------------
 // plans
 // ---------------------------------------------------------------------------
-// forced choice for the autotuner's trial launches (cfg < 0: use the table)
+// forced choice for the autotuner's trial launches: cfg >= 0 forces one choice
+// on every problem; otherwise `over` (bucket -> choice) is consulted before the table
 struct Forced {
   int cfg = -1, splitk = 1;
+  const std::map<std::string, TuneEntry>* over = nullptr;
 };
 
 }  // namespace
@@ -503,7 +505,17 @@ int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& f
       g.cfg = forced.cfg;
       g.splitk = forced.splitk;
     } else {
-      choose(method, g.m, g.n, g.k, g.cfg, g.splitk);
+      const TuneEntry* o = nullptr;
+      if (forced.over) {
+        auto it = forced.over->find(bucket_key(method, g.m, g.n, g.k));
+        if (it != forced.over->end()) o = &it->second;
+      }
+      if (o) {
+        g.cfg = o->cfg;
+        g.splitk = o->splitk;
+      } else {
+        choose(method, g.m, g.n, g.k, g.cfg, g.splitk);
+      }
     }
     const int bm = CFG_BM[g.cfg];
     g.tiles_m = (int)((g.m + bm - 1) / bm);
@@ -741,6 +753,37 @@ int jb_grouped_plan_destroy(jb_grouped_plan* plan) {
   return JB_OK;
 }
 
+// Builds the plan of the whole group with the bucket choices `over`, runs it
+// once to warm up and `reps` times under events; *ms = the fastest run.
+static int autotune_trial(int method, int64_t count, const jb_gemm_desc* descs,
+                          const std::map<std::string, TuneEntry>& over, int reps, hipStream_t s, hipEvent_t e0,
+                          hipEvent_t e1, double* ms) {
+  Forced fc;
+  fc.over = &over;
+  GroupedPlanImpl P;
+  int rc = plan_build(method, count, descs, fc, P);
+  if (rc) {
+    plan_free(P);
+    return rc;
+  }
+  hipError_t e = plan_enqueue(P, s);
+  double best = 1e30;
+  for (int r = 0; r < reps && e == hipSuccess; ++r) {
+    if ((e = hipEventRecord(e0, s)) != hipSuccess) break;
+    if ((e = plan_enqueue(P, s)) != hipSuccess) break;
+    if ((e = hipEventRecord(e1, s)) != hipSuccess) break;
+    if ((e = hipEventSynchronize(e1)) != hipSuccess) break;
+    float t = 0;
+    hipEventElapsedTime(&t, e0, e1);
+    best = std::min(best, (double)t);
+  }
+  hipStreamSynchronize(s);
+  plan_free(P);
+  if (e != hipSuccess) return hip_fail(e, "autotune trial");
+  *ms = best;
+  return JB_OK;
+}
+
 int jb_autotune_grouped(int method, int64_t count, const jb_gemm_desc* descs, int reps, void* stream) {
   int rc = validate_descs(method, count, descs);
   if (rc) return rc;
@@ -748,58 +791,59 @@ int jb_autotune_grouped(int method, int64_t count, const jb_gemm_desc* descs, in
   std::lock_guard<std::mutex> lk(rt().mu);
   if ((rc = ensure_init())) return rc;
   hipStream_t s = (hipStream_t)stream;
-  // one representative problem (the median by work) per bucket
-  std::map<std::string, std::vector<int64_t>> buckets;
-  for (int64_t i = 0; i < count; ++i)
-    if (descs[i].m > 0 && descs[i].n > 0 && descs[i].k > 0)
-      buckets[bucket_key(method, descs[i].m, descs[i].n, descs[i].k)].push_back(i);
+  // Tuning happens in context: every trial times the whole group, so a
+  // bucket's choice is judged by what it does to the group's time (a split-K
+  // that helps a bucket alone can slow a group that already fills the GPU).
+  // Coordinate descent, one pass, buckets in decreasing order of their flops;
+  // a candidate replaces the current choice only if the group gets >= 1 % faster.
+  struct Bucket {
+    double flops = 0;
+    int64_t k = 0;
+  };
+  std::map<std::string, Bucket> buckets;
+  double total = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    const jb_gemm_desc& d = descs[i];
+    if (d.m <= 0 || d.n <= 0 || d.k <= 0) continue;
+    Bucket& b = buckets[bucket_key(method, d.m, d.n, d.k)];
+    const double f = 2.0 * (double)d.m * (double)d.n * (double)d.k;
+    b.flops += f;
+    b.k = std::max<int64_t>(b.k, d.k);
+    total += f;
+  }
+  std::vector<std::pair<double, std::string>> order;
+  for (auto& kv : buckets)
+    if (kv.second.flops >= 0.005 * total) order.push_back({kv.second.flops, kv.first});
+  std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
-  for (auto& kv : buckets) {
-    auto& ids = kv.second;
-    std::sort(ids.begin(), ids.end(), [&](int64_t a, int64_t b) {
-      return (double)descs[a].m * descs[a].n * descs[a].k < (double)descs[b].m * descs[b].n * descs[b].k;
-    });
-    // up to 16 problems of the bucket run together, the way they run in a group
-    std::vector<jb_gemm_desc> sample;
-    for (size_t q = 0; q < ids.size() && sample.size() < 16; q += std::max<size_t>(1, ids.size() / 16))
-      sample.push_back(descs[ids[q]]);
-    TuneEntry best{-1, 1, 1e30};
-    for (int cfg = 0; cfg < NCFG; ++cfg)
-      for (int sk = 1; sk <= 8; sk *= 2) {
-        if (sk > 1 && sample[0].k / sk < 128) continue;
-        Forced fc;
-        fc.cfg = cfg;
-        fc.splitk = sk;
-        GroupedPlanImpl P;
-        if ((rc = plan_build(method, (int64_t)sample.size(), sample.data(), fc, P))) {
-          plan_free(P);
-          hipEventDestroy(e0);
-          hipEventDestroy(e1);
-          return rc;
-        }
-        hipError_t e = plan_enqueue(P, s);  // warm-up
-        hipEventRecord(e0, s);
-        for (int r = 0; r < reps && e == hipSuccess; ++r) e = plan_enqueue(P, s);
-        hipEventRecord(e1, s);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        float ms = 0;
-        if (e == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
-        hipStreamSynchronize(s);
-        plan_free(P);
-        if (e != hipSuccess) {
-          hipEventDestroy(e0);
-          hipEventDestroy(e1);
-          return hip_fail(e, "autotune trial");
-        }
-        const double per = ms / reps;
-        if (per < best.ms) best = TuneEntry{cfg, sk, per};
+  std::map<std::string, TuneEntry> over;
+  double cur = 0;
+  rc = autotune_trial(method, count, descs, over, reps, s, e0, e1, &cur);
+  for (size_t bi = 0; bi < order.size() && rc == JB_OK; ++bi) {
+    const std::string& key = order[bi].second;
+    const int64_t kmax = buckets[key].k;
+    TuneEntry best{-1, 1, cur};
+    for (int cfg = 0; cfg < NCFG && rc == JB_OK; ++cfg)
+      for (int sk = 1; sk <= 8 && rc == JB_OK; sk *= 2) {
+        if (sk > 1 && kmax / sk < 128) continue;
+        over[key] = TuneEntry{cfg, sk, 0.0};
+        double t = 0;
+        rc = autotune_trial(method, count, descs, over, reps, s, e0, e1, &t);
+        if (rc == JB_OK && t < 0.99 * best.ms) best = TuneEntry{cfg, sk, t};
       }
-    g_table[kv.first] = best;
+    if (best.cfg >= 0) {
+      over[key] = best;
+      cur = best.ms;
+    } else {
+      over.erase(key);
+    }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  if (rc) return rc;
+  for (auto& kv : over) g_table[kv.first] = kv.second;
   return JB_OK;
 }
 
@@ -874,6 +918,31 @@ static int pullback_descs(char ta, char tb, int64_t m, int64_t n, int64_t k, dou
   return JB_OK;
 }
 
+// The adjoint GEMMs of one large product fill the GPU on their own: those run
+// on the single-problem kernels (128x128 tiles, stream-ordered one after the
+// other); smaller ones share one grouped launch.
+static bool pullback_big(const jb_gemm_desc& d) {
+  return (double)d.m * (double)d.n >= 256.0 * 128.0 * 128.0 && d.k >= 256;
+}
+
+static int pullback_run_locked(int method, const std::vector<jb_gemm_desc>& d, hipStream_t s) {
+  std::vector<jb_gemm_desc> small;
+  for (const jb_gemm_desc& x : d) {
+    if (!pullback_big(x)) {
+      small.push_back(x);
+      continue;
+    }
+    GemmArgs g;
+    int rc = validate_gemm(x.transA, x.transB, x.m, x.n, x.k, x.alpha == 0.0, x.A, x.lda, x.B, x.ldb, x.C, x.ldc, &g);
+    if (rc) return rc;
+    g.alpha = x.alpha;
+    g.beta = x.beta;
+    if ((rc = single_gemm_dev_locked(method, g, s))) return rc;
+  }
+  if (small.empty()) return JB_OK;
+  return run_grouped_once(method, (int64_t)small.size(), small.data(), Forced(), s);
+}
+
 static int pullback_validate(int method, char ta, char tb, int64_t m, int64_t n, int64_t k, const void* A, int64_t lda,
                              const void* B, int64_t ldb, const void* dC, int64_t lddc, const void* dA, int64_t ldda,
                              const void* dB, int64_t lddb) {
@@ -904,7 +973,10 @@ int jb_gemm_pullback_dev(int method, char transA, char transB, int64_t m, int64_
   std::vector<jb_gemm_desc> d;
   pullback_descs(transA, transB, m, n, k, alpha, A, lda, B, ldb, dC, lddc, dA, ldda, dB, lddb, d);
   if (d.empty()) return JB_OK;
-  return jb_gemm_grouped_dev(method, (int64_t)d.size(), d.data(), stream);
+  if ((rc = validate_descs(method, (int64_t)d.size(), d.data()))) return rc;
+  std::lock_guard<std::mutex> lk(rt().mu);
+  if ((rc = ensure_init())) return rc;
+  return pullback_run_locked(method, d, (hipStream_t)stream);
 }
 
 int jb_gemm_pullback(int method, char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
@@ -943,7 +1015,7 @@ int jb_gemm_pullback(int method, char transA, char transB, int64_t m, int64_t n,
   if (!d.empty()) {
     for (auto& x : d)
       if ((rc = validate_descs(method, 1, &x))) return rc;
-    if ((rc = run_grouped_once(method, (int64_t)d.size(), d.data(), Forced(), s))) return rc;
+    if ((rc = pullback_run_locked(method, d, s))) return rc;
   }
   if (dA) d2h(dA, gA, ar, ac, ldda);
   if (dB) d2h(dB, gB, br, bc, lddb);

@@ -42,6 +42,10 @@ int ws_release(hipStream_t s);
 int device_flag(int** out);
 int run_staged(const GemmArgs& g, size_t elt, bool read_c, const std::function<int(const GemmArgs&, hipStream_t)>& dev);
 
+// One GEMM of a grouped method (JB_F64_NATIVE, JB_F32_NATIVE, JB_F32_BF16X3
+// with six products) on the single-problem kernels.
+int single_gemm_dev_locked(int method, const GemmArgs& g, hipStream_t s);
+
 // grouped / autotuner (jb_grouped.hip)
 void grouped_release_all();
 int tune_load_locked(const char* path);

@@ -553,6 +553,15 @@ static int sgemm_dev_locked(const GemmArgs& g, int method, int nprod, hipStream_
   return rc;
 }
 
+// One problem on the single-problem kernels of a grouped method (the caller
+// holds the lock and has validated `g`).
+extern "C++" int jb::single_gemm_dev_locked(int method, const GemmArgs& g, hipStream_t s) {
+  if (method == JB_F64_NATIVE) return dgemm_dev_locked(g, false, s);
+  if (method == JB_F32_NATIVE) return sgemm_dev_locked(g, JB_F32_NATIVE, 0, s);
+  if (method == JB_F32_BF16X3) return sgemm_dev_locked(g, JB_F32_BF16X3, 6, s);
+  return fail(-1, "unknown method %d", method);
+}
+
 int jb_sgemm_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
                  const float* B, int64_t ldb, float beta, float* C, int64_t ldc, void* stream) {
   JB_GEMM_PROLOGUE(alpha, beta)

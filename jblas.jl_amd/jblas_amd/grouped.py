@@ -103,7 +103,8 @@ class GroupedPlan:
 
 # ---- autotuner ---------------------------------------------------------------
 def autotune(method, descs: Sequence[GemmDesc], reps: int = 3, *, stream=None) -> None:
-    """Time every (tile, split-K) candidate per size bucket present in `descs`."""
+    """Tune (tile, split-K) per size bucket present in `descs`, timing the
+    whole group for every candidate (coordinate descent over buckets)."""
     arr = _desc_array(descs)
     _lib.call("jb_autotune_grouped", method_id(method), len(descs), C.cast(arr, C.c_void_p), reps,
               _stream_of(True, stream))
