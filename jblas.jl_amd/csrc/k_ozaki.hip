@@ -265,7 +265,8 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   constexpr int PLANE_B = (QBM + QBN) * QBK * 2;  // bytes of one plane in one stage (6 KB)
   constexpr int STAGE_B = S * PLANE_B;
   constexpr int NWL = 6 * S;                      // 1 KB wave-loads per stage
-  constexpr int WL_MIN = NWL / NWAVES;            // fewest wave-loads any wave issues per stage
+  constexpr int WL_MIN = NWL / NWAVES;            // wave-loads per stage of waves >= NWL % NWAVES
+  constexpr int WL_MAX = (NWL + NWAVES - 1) / NWAVES;  // ... of waves < NWL % NWAVES
   constexpr int NL = (MODE == 0) ? S : 2;         // FP32 accumulator sets
   constexpr int NG = (S + 3) / 4;                 // FP64 groups (MODE 0)
   constexpr int KC_ST = chunk_len(S) / QBK;       // stages per exact chunk
@@ -342,7 +343,11 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   issue(0, 0);
   if (nst > 1) issue(1, 1);
   for (int st = 0; st < nst; ++st) {
-    if (st + 1 < nst) wait_vmcnt<WL_MIN>();  // this stage landed; the next may be in flight
+    // this stage landed; the next stage's loads (this wave's count) may be in flight
+    if (st + 1 < nst) {
+      if (wave < NWL % NWAVES) wait_vmcnt<WL_MAX>();
+      else wait_vmcnt<WL_MIN>();
+    }
     else wait_vmcnt<0>();
     block_barrier();  // every wave's loads of stage st landed, every wave is done with st-1
     if (st + 2 < nst) issue(st + 2, (st + 2) % QSLOTS);
