@@ -155,6 +155,19 @@ int jb_dgemm_ozaki_dev(char transA, char transB, int64_t m, int64_t n, int64_t k
                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                        double* C, int64_t ldc, int nslices, void* stream);
 
+/* Emulated FP64 on INT8 slices: the same digit weights (2^(-7-8t) times the
+ * row/column exponent) with each digit a signed byte of the operand rounded to
+ * 8*nslices-1 bits below its row/column scale, slice pairs (t+u <= nslices+1)
+ * on v_mfma_i32_32x32x32_i8 with exact int32 level sums (k-chunks of 8192),
+ * level sums combined in compensated FP64 as above.  No counterpart in the
+ * reference (the drop-in's faster emulated method); nslices 1..8 (argument 14). */
+int jb_dgemm_ozaki_i8(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
+                      const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                      double* C, int64_t ldc, int nslices);
+int jb_dgemm_ozaki_i8_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
+                          const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                          double* C, int64_t ldc, int nslices, void* stream);
+
 /* FP32 on FP32 MFMA. */
 int jb_sgemm(char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha,
              const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,

@@ -40,6 +40,9 @@ size_t ozaki_workspace_bytes(int64_t m, int64_t n, int64_t k, int s);
 // ozaki_workspace_bytes(); `flag` is a device int that the split kernels set
 // to 1 on a non-finite input (caller zeroes it first).
 hipError_t launch_dgemm_ozaki(const GemmArgs& g, int s, void* ws, int* flag, hipStream_t st);
+// The same with INT8 slices (v_mfma_i32_32x32x32_i8, exact int32 level sums).
+size_t ozaki_i8_workspace_bytes(int64_t m, int64_t n, int64_t k, int s);
+hipError_t launch_dgemm_ozaki_i8(const GemmArgs& g, int s, void* ws, int* flag, hipStream_t st);
 
 // ---- FP32 -----------------------------------------------------------------
 hipError_t launch_sgemm_mfma(const GemmArgs& g, hipStream_t s);

@@ -487,8 +487,8 @@ int jb_fastmul_f32_dev(int64_t M, int64_t N, int64_t P, float* D, const float* A
   return fastmul_entry<float>(M, N, P, D, A, X, stream, false);
 }
 
-// ---- FP64 emulated (split-BF16) -------------------------------------------
-static int ozaki_dev_locked(const GemmArgs& g, int s, hipStream_t st) {
+// ---- FP64 emulated (split-BF16 / split-INT8) -------------------------------
+static int ozaki_dev_locked(const GemmArgs& g, int s, bool i8, hipStream_t st) {
   if (g.m == 0 || g.n == 0) return JB_OK;
   if (g.alpha == 0.0 || g.k == 0) {
     hipError_t e = launch_scale_f64(g.m, g.n, g.beta, static_cast<double*>(g.C), g.ldc, st);
@@ -498,10 +498,10 @@ static int ozaki_dev_locked(const GemmArgs& g, int s, hipStream_t st) {
   int rc = device_flag(&flag);
   if (rc) return rc;
   void* ws = nullptr;
-  rc = ws_acquire(ozaki_workspace_bytes(g.m, g.n, g.k, s), st, &ws);
+  rc = ws_acquire(i8 ? ozaki_i8_workspace_bytes(g.m, g.n, g.k, s) : ozaki_workspace_bytes(g.m, g.n, g.k, s), st, &ws);
   if (rc) return rc;
   hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), st);
-  if (e == hipSuccess) e = launch_dgemm_ozaki(g, s, ws, flag, st);
+  if (e == hipSuccess) e = i8 ? launch_dgemm_ozaki_i8(g, s, ws, flag, st) : launch_dgemm_ozaki(g, s, ws, flag, st);
   int relrc = ws_release(st);
   if (e != hipSuccess) return hip_fail(e, "dgemm_ozaki launch");
   if (relrc) return relrc;
@@ -520,7 +520,7 @@ int jb_dgemm_ozaki_dev(char transA, char transB, int64_t m, int64_t n, int64_t k
                        void* stream) {
   if (nslices < 1 || nslices > 8) return fail(-14, "nslices must be in 1..8 (got %d)", nslices);
   JB_GEMM_PROLOGUE(alpha, beta)
-  return ozaki_dev_locked(g, nslices, (hipStream_t)stream);
+  return ozaki_dev_locked(g, nslices, false, (hipStream_t)stream);
 }
 
 int jb_dgemm_ozaki(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
@@ -528,7 +528,24 @@ int jb_dgemm_ozaki(char transA, char transB, int64_t m, int64_t n, int64_t k, do
   if (nslices < 1 || nslices > 8) return fail(-14, "nslices must be in 1..8 (got %d)", nslices);
   JB_GEMM_PROLOGUE(alpha, beta)
   return run_staged(g, sizeof(double), beta != 0.0, [nslices](const GemmArgs& d, hipStream_t s) {
-    return ozaki_dev_locked(d, nslices, s);
+    return ozaki_dev_locked(d, nslices, false, s);
+  });
+}
+
+int jb_dgemm_ozaki_i8_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                          int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int nslices,
+                          void* stream) {
+  if (nslices < 1 || nslices > 8) return fail(-14, "nslices must be in 1..8 (got %d)", nslices);
+  JB_GEMM_PROLOGUE(alpha, beta)
+  return ozaki_dev_locked(g, nslices, true, (hipStream_t)stream);
+}
+
+int jb_dgemm_ozaki_i8(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                      int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int nslices) {
+  if (nslices < 1 || nslices > 8) return fail(-14, "nslices must be in 1..8 (got %d)", nslices);
+  JB_GEMM_PROLOGUE(alpha, beta)
+  return run_staged(g, sizeof(double), beta != 0.0, [nslices](const GemmArgs& d, hipStream_t s) {
+    return ozaki_dev_locked(d, nslices, true, s);
   });
 }
 

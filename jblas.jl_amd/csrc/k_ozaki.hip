@@ -30,6 +30,7 @@
 //   to nearest of the running residual, exact), products t+u <= LMAX, level 2
 //   in one FP32 accumulator and the smaller levels in a second.
 #include <algorithm>
+#include <type_traits>
 
 #include "jb_internal.h"
 
@@ -39,6 +40,8 @@ namespace {
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+typedef int i4v __attribute__((ext_vector_type(4)));
+typedef int i16v __attribute__((ext_vector_type(16)));
 #define JB_LDS __attribute__((address_space(3)))
 
 constexpr int QBM = 128, QBN = 64;  // block tile (4/WB x 2 waves of 32*WB x 32)
@@ -168,6 +171,70 @@ __global__ __launch_bounds__(256) void split_f64_kernel(const double* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// 2c. FP64 split into S INT8 digit planes (+ exponents).  E = frexp exponent of
+//     the row max, plus one when max*2^-E > 126.5/128; X = rint(x * 2^(8S-1))
+//     with x = ldexp(a, -E); the digits are X's signed base-256 digits (least
+//     significant from plane S-1 up, each in [-128, 127]), so that
+//     a ~= 2^E * sum_t d_t 2^(-7-8t), the weights of the BF16 split.  A plane
+//     stage is 32 int8 of k per row (one v_mfma_i32_32x32x32_i8 k-step): its
+//     byte image has the shape of a BF16 stage of 16.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t plane_off8(int64_t row, int64_t k, int RB, int64_t Kpad) {
+  return (row / RB) * (RB * Kpad) + (k / 32) * (RB * 32) + (row % RB) * 32 + (k % 32);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void split_f64_i8_kernel(const double* __restrict__ P, int64_t sr, int64_t sk,
+                                                           int64_t R, int64_t K,
+                                                           const unsigned long long* __restrict__ maxbits,
+                                                           int* __restrict__ exps, uint8_t* __restrict__ planes,
+                                                           int64_t plane_stride, int64_t Kpad, int RB,
+                                                           int* __restrict__ flag) {
+  __shared__ double tile[64][65];
+  const int t = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
+  const bool rows_contig = (sr == 1);
+#pragma unroll 4
+  for (int j = 0; j < 16; ++j) {
+    const int r = rows_contig ? (t % 64) : (t / 64 + 4 * j);
+    const int k = rows_contig ? (t / 64 + 4 * j) : (t % 64);
+    tile[r][k] = (r0 + r < R && k0 + k < K) ? P[(r0 + r) * sr + (k0 + k) * sk] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int grp = t + 256 * q;  // 512 groups of 8 consecutive k
+    const int row = grp / 8, kg = 8 * (grp % 8);
+    int E = 0;
+    if (r0 + row < R) {
+      const unsigned long long mb = maxbits[r0 + row];
+      if (mb >= 0x7FF0000000000000ull) {
+        *flag = 1;  // Inf or NaN in this row/column
+      } else if (mb != 0ull) {
+        E = exponent_of_bits(mb);
+        if (ldexp(__longlong_as_double((long long)mb), -E) > 126.5 / 128.0) E += 1;
+      }
+      if (blockIdx.y == 0 && kg == 0) exps[r0 + row] = E;
+    }
+    long long X[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) X[e] = (long long)rint(ldexp(tile[row][kg + e], 8 * S - 1 - E));
+    const int64_t off = plane_off8(r0 + row, k0 + kg, RB, Kpad);
+#pragma unroll
+    for (int s = S - 1; s >= 0; --s) {
+      uint32_t w[2] = {0u, 0u};
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const long long d = (s > 0) ? (long long)(int8_t)(uint8_t)(X[e] & 255) : X[e];
+        if (s > 0) X[e] = (X[e] - d) >> 8;
+        w[e / 4] |= ((uint32_t)(uint8_t)(int8_t)d) << (8 * (e % 4));
+      }
+      *reinterpret_cast<uint2*>(planes + s * plane_stride + off) = uint2{w[0], w[1]};
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 2b. FP32 split into 3 BF16 planes: a1 = bf16(a), a2 = bf16(a - a1), a3 = bf16(rest).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint16_t bf16_rn_bits(float f) {
@@ -219,8 +286,10 @@ __global__ __launch_bounds__(256) void split_f32_kernel(const float* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// 3. slice-pair GEMM.  MODE 0: emulated FP64 (S slices, exact level sums, FP64
-//    groups).  MODE 1: 3xBF16 FP32 (S = 3, pairs t+u <= LMAX).
+// 3. slice-pair GEMM.  MODE 0: emulated FP64 (S BF16 slices, exact FP32 level
+//    sums, FP64 groups).  MODE 1: 3xBF16 FP32 (S = 3, pairs t+u <= LMAX).
+//    MODE 2: emulated FP64 on S INT8 slices (v_mfma_i32_32x32x32_i8, exact
+//    int32 level sums, 32 k per stage, same LDS image and combine as MODE 0).
 // ---------------------------------------------------------------------------
 struct PairArgs {
   const uint16_t* pa;  // op(A) planes, row block 128
@@ -267,9 +336,12 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   constexpr int NWL = 6 * S;                      // 1 KB wave-loads per stage
   constexpr int WL_MIN = NWL / NWAVES;            // wave-loads per stage of waves >= NWL % NWAVES
   constexpr int WL_MAX = (NWL + NWAVES - 1) / NWAVES;  // ... of waves < NWL % NWAVES
-  constexpr int NL = (MODE == 0) ? S : 2;         // FP32 accumulator sets
-  constexpr int NG = (S + 3) / 4;                 // FP64 groups (MODE 0)
-  constexpr int KC_ST = chunk_len(S) / QBK;       // stages per exact chunk
+  constexpr int NL = (MODE == 1) ? 2 : S;         // accumulator sets (levels)
+  constexpr int NG = (S + 3) / 4;                 // FP64 groups (MODES 0, 2)
+  // stages per exact chunk: FP32 level sums below 2^24 (MODE 0); int32 sums of
+  // 8192 k (|S_l| <= 8 * 8192 * 2^14 = 2^30) in MODE 2, whose stages hold 32 k
+  constexpr int KC_ST = (MODE == 2) ? 8192 / 32 : chunk_len(S) / QBK;
+  typedef typename std::conditional<MODE == 2, i16v, f16v>::type AccV;
   __shared__ __attribute__((aligned(1024))) char ring[QSLOTS * STAGE_B];
 
   const int nblocks = p.tiles_m * p.tiles_n;
@@ -293,36 +365,38 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   const int dma_off = (lane / 2) * QBK + 8 * ((lane % 2) ^ ((lane >> 4) & 1));
   const uint16_t* gA = p.pa + (int64_t)tm * QBM * p.Kpad + dma_off;
   const uint16_t* gB = p.pb + (int64_t)tn * QBN * p.Kpad + dma_off;
-  auto issue = [&](int ks, int slot) {
-    char* base = ring + slot * STAGE_B;
-#pragma unroll
-    for (int i = 0; i < (NWL + NWAVES - 1) / NWAVES; ++i) {
-      const int j = wave + NWAVES * i;  // uniform
-      if (j >= NWL) break;
-      const uint16_t* src;
-      int loff;
-      if (j < 4 * S) {
-        const int t = j / 4, q = j % 4;
-        src = gA + t * p.strideA + (int64_t)ks * (QBM * QBK) + q * 512;
-        loff = t * PLANE_B + q * 1024;
-      } else {
-        const int jj = j - 4 * S, t = jj / 2, q = jj % 2;
-        src = gB + t * p.strideB + (int64_t)ks * (QBN * QBK) + q * 512;
-        loff = t * PLANE_B + QBM * QBK * 2 + q * 1024;
-      }
-      __builtin_amdgcn_global_load_lds((const void*)src, (JB_LDS void*)(base + loff), 16, 0, 0);
+  // Piece i of this wave's share of stage ks (pieces j = wave + NWAVES*i; A:
+  // plane t, 4 pieces of 1 KB; B: plane t, 2 pieces) -> LDS slot `slot`.
+  auto issue_piece = [&](int ks, int slot, int i) {
+    const int j = wave + NWAVES * i;  // uniform
+    if (j >= NWL) return;
+    const uint16_t* src;
+    int loff;
+    if (j < 4 * S) {
+      const int t = j / 4, q = j % 4;
+      src = gA + t * p.strideA + (int64_t)ks * (QBM * QBK) + q * 512;
+      loff = t * PLANE_B + q * 1024;
+    } else {
+      const int jj = j - 4 * S, t = jj / 2, q = jj % 2;
+      src = gB + t * p.strideB + (int64_t)ks * (QBN * QBK) + q * 512;
+      loff = t * PLANE_B + QBM * QBK * 2 + q * 1024;
     }
+    __builtin_amdgcn_global_load_lds((const void*)src, (JB_LDS void*)(ring + slot * STAGE_B + loff), 16, 0, 0);
+  };
+  auto issue = [&](int ks, int slot) {
+#pragma unroll
+    for (int i = 0; i < WL_MAX; ++i) issue_piece(ks, slot, i);
   };
 
-  f16v acc[NL][WB];
+  AccV acc[NL][WB];
 #pragma unroll
   for (int l = 0; l < NL; ++l)
 #pragma unroll
     for (int b = 0; b < WB; ++b)
 #pragma unroll
-      for (int e = 0; e < 16; ++e) acc[l][b][e] = 0.0f;
-  double G[MODE == 0 ? NG : 1][WB][16];
-  if (MODE == 0) {
+      for (int e = 0; e < 16; ++e) acc[l][b][e] = 0;
+  double G[MODE != 1 ? NG : 1][WB][16];
+  if (MODE != 1) {
 #pragma unroll
     for (int g = 0; g < NG; ++g)
 #pragma unroll
@@ -350,9 +424,13 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
     }
     else wait_vmcnt<0>();
     block_barrier();  // every wave's loads of stage st landed, every wave is done with st-1
-    if (st + 2 < nst) issue(st + 2, (st + 2) % QSLOTS);
+    // Stage st+2 goes into the slot of st-1, one piece per step below: a burst of
+    // loads right after the barrier fills the memory pipeline's queue and holds
+    // up the wave's MFMA issue behind it.
+    const bool prefetch = st + 2 < nst;
+    const int pslot = (st + 2) % QSLOTS;
     const u4* L = reinterpret_cast<const u4*>(ring + (st % QSLOTS) * STAGE_B);
-    const bool chunk_end = (MODE == 0) && ((((st + 1) % KC_ST) == 0) || (st + 1 == nst));
+    const bool chunk_end = (MODE != 1) && ((((st + 1) % KC_ST) == 0) || (st + 1 == nst));
     // Pairs run in square order: step j multiplies the new slices A_j and B_j
     // with all earlier ones (pairs with max(t, u) == j).  A step needs only two
     // new fragments, read one step ahead, so a stage starts after two LDS reads
@@ -361,8 +439,13 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
 #pragma unroll
     for (int b = 0; b < WB; ++b) af[0][b] = __builtin_bit_cast(bf8, L[(wm0 + 32 * b + l32) * 2 + phl]);
     bf[0] = __builtin_bit_cast(bf8, L[(QBM + wn0 + l32) * 2 + phl]);
+    constexpr int PPS = (WL_MAX + S - 1) / S;  // load pieces per step
 #pragma unroll
     for (int j = 0; j < S; ++j) {
+      if (prefetch) {
+#pragma unroll
+        for (int i = PPS * j; i < PPS * (j + 1) && i < WL_MAX; ++i) issue_piece(st + 2, pslot, i);
+      }
       if (j + 1 < S) {
 #pragma unroll
         for (int b = 0; b < WB; ++b)
@@ -374,14 +457,18 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
         const int t = q <= j ? j : q - j - 1;  // (j, 0..j), then (0..j-1, j)
         const int u = q <= j ? q : j;
         if (t + u + 2 > LMAX) continue;
-        const int l = (MODE == 0) ? (t + u) : (t + u == 0 ? 0 : 1);
+        const int l = (MODE != 1) ? (t + u) : (t + u == 0 ? 0 : 1);
         // X = op(B) plane (p -> n), Y = op(A) plane (q -> m): item r of lane l holds
         // C[m0+wm0+32b+l%32][n0+wn0+8(r/4)+4(l/32)+r%4].
 #pragma unroll
         for (int b = 0; b < WB; ++b)
-          acc[l][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bf[u], af[t][b], acc[l][b], 0, 0, 0);
+          if constexpr (MODE == 2)
+            acc[l][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(__builtin_bit_cast(i4v, bf[u]),
+                                                              __builtin_bit_cast(i4v, af[t][b]), acc[l][b], 0, 0, 0);
+          else
+            acc[l][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bf[u], af[t][b], acc[l][b], 0, 0, 0);
       }
-      if (MODE == 0 && chunk_end && j < NL) {
+      if (MODE != 1 && chunk_end && j < NL) {
         // level j has received its last product of this chunk: flush it exactly
         constexpr double kScale[4] = {1.0, 0x1.0p-8, 0x1.0p-16, 0x1.0p-24};
 #pragma unroll
@@ -389,13 +476,13 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
 #pragma unroll
           for (int e = 0; e < 16; ++e) {
             G[j / 4][b][e] = fma((double)acc[j][b][e], kScale[j % 4], G[j / 4][b][e]);
-            acc[j][b][e] = 0.0f;
+            acc[j][b][e] = 0;
           }
       }
     }
   }
 
-  if (MODE == 0) {
+  if (MODE != 1) {
     double* C = static_cast<double*>(eC);
 #pragma unroll
     for (int b = 0; b < WB; ++b) {
@@ -450,7 +537,8 @@ struct Layout {
   size_t offA, offB, offMA, offMB, offEA, offEB, total;
 };
 
-Layout make_layout(int64_t m, int64_t n, int64_t k, int s) {
+// elt: bytes per digit (2: BF16 planes, 1: INT8 planes).
+Layout make_layout(int64_t m, int64_t n, int64_t k, int s, int elt = 2) {
   Layout L;
   L.Mpad = rup(std::max<int64_t>(m, 1), QBM);
   L.Npad = rup(std::max<int64_t>(n, 1), QBN);
@@ -458,9 +546,9 @@ Layout make_layout(int64_t m, int64_t n, int64_t k, int s) {
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   size_t off = 0;
   L.offA = off;
-  off = al(off + (size_t)s * L.Mpad * L.Kpad * 2);
+  off = al(off + (size_t)s * L.Mpad * L.Kpad * elt);
   L.offB = off;
-  off = al(off + (size_t)s * L.Npad * L.Kpad * 2);
+  off = al(off + (size_t)s * L.Npad * L.Kpad * elt);
   L.offMA = off;
   off = al(off + (size_t)L.Mpad * 8);
   L.offMB = off;
@@ -495,14 +583,16 @@ void launch_split_f64(const double* P, int64_t sr, int64_t sk, int64_t R, int64_
                      sk, R, K, mb, ex, planes, Rpad * Kpad, Kpad, RB, flag);
 }
 
+// elt as in make_layout: the kernel addresses planes in 16-bit units, so an
+// INT8 plane of Kpad k is a BF16-shaped plane of Kpad/2.
 PairArgs pair_args(const GemmArgs& g, const Layout& L, const uint16_t* pA, const uint16_t* pB, const int* eA,
-                   const int* eB) {
+                   const int* eB, int elt = 2) {
   PairArgs p;
   p.pa = pA;
   p.pb = pB;
-  p.Kpad = L.Kpad;
-  p.strideA = L.Mpad * L.Kpad;
-  p.strideB = L.Npad * L.Kpad;
+  p.Kpad = L.Kpad * elt / 2;
+  p.strideA = L.Mpad * p.Kpad;
+  p.strideB = L.Npad * p.Kpad;
   p.eA = eA;
   p.eB = eB;
   p.M = g.m;
@@ -540,9 +630,66 @@ hipError_t run_ozaki(const GemmArgs& g, const Layout& L, char* ws, int* flag, hi
   return hipGetLastError();
 }
 
+template <int S>
+void launch_split_f64_i8(const double* P, int64_t sr, int64_t sk, int64_t R, int64_t K, int64_t Rpad, int64_t Kpad,
+                         int RB, unsigned long long* mb, int* ex, uint8_t* planes, int* flag, hipStream_t st) {
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)std::min<int64_t>((Rpad + 255) / 256, 1024)), dim3(256), 0, st,
+                     mb, Rpad);
+  hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)((R + 63) / 64), (unsigned)((K + 63) / 64)), dim3(256), 0, st, P,
+                     sr, sk, R, K, mb);
+  hipLaunchKernelGGL(split_f64_i8_kernel<S>, dim3((unsigned)(Rpad / 64), (unsigned)(Kpad / 64)), dim3(256), 0, st, P,
+                     sr, sk, R, K, mb, ex, planes, Rpad * Kpad, Kpad, RB, flag);
+}
+
+template <int S>
+hipError_t run_ozaki_i8(const GemmArgs& g, const Layout& L, char* ws, int* flag, hipStream_t st) {
+  int64_t sr, sk;
+  auto* mbA = reinterpret_cast<unsigned long long*>(ws + L.offMA);
+  auto* mbB = reinterpret_cast<unsigned long long*>(ws + L.offMB);
+  auto* eA = reinterpret_cast<int*>(ws + L.offEA);
+  auto* eB = reinterpret_cast<int*>(ws + L.offEB);
+  auto* pA = reinterpret_cast<uint8_t*>(ws + L.offA);
+  auto* pB = reinterpret_cast<uint8_t*>(ws + L.offB);
+  timing_begin(st, "ozaki_i8_split");
+  strides_a(g.ta, g.lda, sr, sk);
+  launch_split_f64_i8<S>(static_cast<const double*>(g.A), sr, sk, g.m, g.k, L.Mpad, L.Kpad, QBM, mbA, eA, pA, flag,
+                         st);
+  strides_b(g.tb, g.ldb, sr, sk);
+  launch_split_f64_i8<S>(static_cast<const double*>(g.B), sr, sk, g.n, g.k, L.Npad, L.Kpad, QBN, mbB, eB, pB, flag,
+                         st);
+  timing_end(st, "ozaki_i8_split");
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const PairArgs p = pair_args(g, L, reinterpret_cast<const uint16_t*>(pA), reinterpret_cast<const uint16_t*>(pB), eA,
+                               eB, 1);
+  timing_begin(st, "slice_pair_gemm_i8");
+  hipLaunchKernelGGL((slice_pair_gemm_kernel<2, S, S + 1, OZ_WB>), dim3(p.tiles_m * p.tiles_n), dim3(512 / OZ_WB), 0,
+                     st, p);
+  timing_end(st, "slice_pair_gemm_i8");
+  return hipGetLastError();
+}
+
 }  // namespace
 
 size_t ozaki_workspace_bytes(int64_t m, int64_t n, int64_t k, int s) { return make_layout(m, n, k, s).total; }
+
+size_t ozaki_i8_workspace_bytes(int64_t m, int64_t n, int64_t k, int s) { return make_layout(m, n, k, s, 1).total; }
+
+hipError_t launch_dgemm_ozaki_i8(const GemmArgs& g, int s, void* ws, int* flag, hipStream_t st) {
+  const Layout L = make_layout(g.m, g.n, g.k, s, 1);
+  char* w = static_cast<char*>(ws);
+  switch (s) {
+    case 1: return run_ozaki_i8<1>(g, L, w, flag, st);
+    case 2: return run_ozaki_i8<2>(g, L, w, flag, st);
+    case 3: return run_ozaki_i8<3>(g, L, w, flag, st);
+    case 4: return run_ozaki_i8<4>(g, L, w, flag, st);
+    case 5: return run_ozaki_i8<5>(g, L, w, flag, st);
+    case 6: return run_ozaki_i8<6>(g, L, w, flag, st);
+    case 7: return run_ozaki_i8<7>(g, L, w, flag, st);
+    case 8: return run_ozaki_i8<8>(g, L, w, flag, st);
+    default: return hipErrorInvalidValue;
+  }
+}
 
 hipError_t launch_dgemm_ozaki(const GemmArgs& g, int s, void* ws, int* flag, hipStream_t st) {
   const Layout L = make_layout(g.m, g.n, g.k, s);

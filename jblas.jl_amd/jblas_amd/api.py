@@ -100,7 +100,7 @@ def _check_shapes(tA, tB, a: _Operand, b: _Operand, c: _Operand):
     return m, n, k
 
 
-METHODS_F64 = ("native", "recurrence", "ozaki")
+METHODS_F64 = ("native", "recurrence", "ozaki", "ozaki_int8")
 METHODS_F32 = ("native", "bf16x3")
 
 
@@ -109,7 +109,8 @@ def gemm_(tA: str, tB: str, alpha, A, B, beta, C, *, method: str = "native", nsl
     """C = alpha*op(A)*op(B) + beta*C in place; returns C.
 
     method (Float64): "native" FP64 MFMA, "recurrence" (bit-exact jmul! order),
-    "ozaki" (split-BF16 emulation with `nslices` slices).
+    "ozaki" (split-BF16 emulation with `nslices` slices), "ozaki_int8" (the
+    same digit weights on INT8 slices and int8 MFMA).
     method (Float32): "native" FP32 MFMA, "bf16x3" (3 BF16 slices, `nprod` products).
     """
     a, b, c = _Operand(A, False, "A"), _Operand(B, False, "B"), _Operand(C, True, "C")
@@ -125,8 +126,8 @@ def gemm_(tA: str, tB: str, alpha, A, B, beta, C, *, method: str = "native", nsl
         elif method == "recurrence":
             name = "jb_dgemm_recurrence_dev" if dev else "jb_dgemm_recurrence"
             args = common + (float(alpha), a.ptr, a.ld, b.ptr, b.ld, float(beta), c.ptr, c.ld)
-        elif method == "ozaki":
-            name = "jb_dgemm_ozaki_dev" if dev else "jb_dgemm_ozaki"
+        elif method in ("ozaki", "ozaki_int8"):
+            name = "jb_dgemm_ozaki" + ("_i8" if method == "ozaki_int8" else "") + ("_dev" if dev else "")
             args = common + (float(alpha), a.ptr, a.ld, b.ptr, b.ld, float(beta), c.ptr, c.ld, int(nslices))
         else:
             raise ValueError(f"Float64 method must be one of {METHODS_F64}")

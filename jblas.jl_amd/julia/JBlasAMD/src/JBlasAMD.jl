@@ -9,7 +9,7 @@ jBLAS.jl's GEMM entry point.  Drop-ins:
     accepts any strided `Matrix{Float64}` (the reference takes `MMatrix`).
   * `gemm!(tA, tB, α, A, B, β, C; method)` -- LinearAlgebra.BLAS.gemm!
     argument order; Float64 methods `:native` (FP64 MFMA), `:recurrence`,
-    `:ozaki` (split-BF16, keyword `nslices`); Float32 `:native`, `:bf16x3`.
+    `:ozaki` (split-BF16, keyword `nslices`), `:ozaki_int8` (INT8 slices); Float32 `:native`, `:bf16x3`.
   * `fastmul!(D, A, X)` -- the same recurrence as one small-GEMM launch
     (src/kernels.jl:202-208), Float64 or Float32, dense matrices.
   * `kernel!(D, A, X)` / `initkernel!(D, A, X)` -- D += A*X / D = A*X on
@@ -107,8 +107,12 @@ function gemm!(tA::AbstractChar, tB::AbstractChar, α::Real, A::StridedMatrix{Fl
         check(:jb_dgemm_ozaki, ccall((:jb_dgemm_ozaki, libjblas), Cint,
             (Cchar, Cchar, Int64, Int64, Int64, Float64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Float64,
              Ptr{Float64}, Int64, Cint), tA, tB, m, n, k, α, A, ld(A), B, ld(B), β, C, ld(C), nslices))
+    elseif method === :ozaki_int8
+        check(:jb_dgemm_ozaki_i8, ccall((:jb_dgemm_ozaki_i8, libjblas), Cint,
+            (Cchar, Cchar, Int64, Int64, Int64, Float64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Float64,
+             Ptr{Float64}, Int64, Cint), tA, tB, m, n, k, α, A, ld(A), B, ld(B), β, C, ld(C), nslices))
     else
-        throw(ArgumentError("Float64 method must be :native, :recurrence or :ozaki"))
+        throw(ArgumentError("Float64 method must be :native, :recurrence, :ozaki or :ozaki_int8"))
     end
     C
 end

@@ -471,9 +471,8 @@ ORC_API int orc_ozaki_split_b(char tb, int64_t k, int64_t n, const double* B, in
   return 0;
 }
 
-static double ozaki_entry(int64_t m, int64_t n, int64_t k, int s, const int* eA, const int16_t* dA,
+static double ozaki_entry(int64_t m, int64_t n, int64_t k, int s, int64_t kc, const int* eA, const int16_t* dA,
                           const int* eB, const int16_t* dB, int64_t i, int64_t j) {
-  const int kc = orc_ozaki_chunk(s);
   const int ngroups = (s + 3) / 4; /* levels 2..s+1 -> groups of 4 */
   double G[4] = {0.0, 0.0, 0.0, 0.0};
   for (int64_t c0 = 0; c0 < k; c0 += kc) {
@@ -524,7 +523,109 @@ ORC_API int orc_dgemm_ozaki(char ta, char tb, int64_t m, int64_t n, int64_t k, d
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
   for (int64_t j = 0; j < n; ++j)
     for (int64_t i = 0; i < m; ++i) {
-      double acc = ozaki_entry(m, n, k, s, eA, dA, eB, dB, i, j);
+      double acc = ozaki_entry(m, n, k, s, orc_ozaki_chunk(s), eA, dA, eB, dB, i, j);
+      double c = alpha * acc;
+      if (beta != 0.0) c = fma(beta, C[i + j * ldc], c);
+      C[i + j * ldc] = c;
+    }
+done:
+  free(eA); free(eB); free(dA); free(dB);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 5b. Split-INT8 emulated DGEMM, exact restatement                          */
+/* ------------------------------------------------------------------------ */
+/*
+ * The same scheme with every digit an int8 (v_mfma_i32_32x32x32_i8: exact
+ * int32 sums), as this build defines it:
+ *   E_i = frexp exponent of max_p |op(A)[i,p]|, plus one if that max scaled
+ *         by 2^-E_i exceeds 126.5/128 (so that the top digit fits in int8);
+ *   X = rint(ldexp(a, -E_i) * 2^(8s-1))   (an integer, |X| < 2^(8s-1));
+ *   the s digits are X's signed base-256 digits, least significant first:
+ *     d_s = (int8)(X & 255), X = (X - d_s)/256, ..., d_1 = X   (all in [-128, 127]);
+ *   so a ~= 2^E_i * sum_t d_t * 2^(-7-8(t-1)), the weights of the BF16 split.
+ * Level sums as above, over chunks of kc = 8192 (|S_l| <= s * kc * 2^14 <= 2^30:
+ * exact in int32), flushed into the FP64 groups in the same order; the
+ * combine and epilogue are those of the BF16 restatement.
+ */
+#define ORC_I8_CHUNK 8192
+ORC_API int orc_ozaki_i8_chunk(void) { return ORC_I8_CHUNK; }
+
+static int row_exponent_i8(const double* v, int64_t n, int64_t stride, int* e) {
+  double mx = 0.0;
+  for (int64_t p = 0; p < n; ++p) {
+    double a = fabs(v[p * stride]);
+    if (!isfinite(a)) return 1;
+    if (a > mx) mx = a;
+  }
+  int ex = 0;
+  if (mx > 0.0) {
+    double f = frexp(mx, &ex);
+    if (f > 126.5 / 128.0) ex += 1;
+  }
+  *e = ex;
+  return 0;
+}
+
+static void split_digits_i8(double a, int e, int s, int16_t* d, int64_t dstride) {
+  int64_t X = (int64_t)rint(ldexp(a, 8 * s - 1 - e));
+  for (int t = s - 1; t >= 1; --t) {
+    const int64_t lo = (int64_t)(int8_t)(uint8_t)(X & 255);
+    d[t * dstride] = (int16_t)lo;
+    X = (X - lo) / 256;
+  }
+  d[0] = (int16_t)X;
+}
+
+ORC_API int orc_ozaki_i8_split_a(char ta, int64_t m, int64_t k, const double* A, int64_t lda, int s,
+                                 int* eA, int16_t* dA) {
+  const int an = (ta == 'N' || ta == 'n');
+  for (int64_t i = 0; i < m; ++i) {
+    int e;
+    if (row_exponent_i8(an ? A + i : A + i * lda, k, an ? lda : 1, &e)) return 1;
+    eA[i] = e;
+    for (int64_t p = 0; p < k; ++p)
+      split_digits_i8(an ? A[i + p * lda] : A[p + i * lda], e, s, dA + i * k + p, m * k);
+  }
+  return 0;
+}
+ORC_API int orc_ozaki_i8_split_b(char tb, int64_t k, int64_t n, const double* B, int64_t ldb, int s,
+                                 int* eB, int16_t* dB) {
+  const int bn = (tb == 'N' || tb == 'n');
+  for (int64_t j = 0; j < n; ++j) {
+    int e;
+    if (row_exponent_i8(bn ? B + j * ldb : B + j, k, bn ? 1 : ldb, &e)) return 1;
+    eB[j] = e;
+    for (int64_t p = 0; p < k; ++p)
+      split_digits_i8(bn ? B[p + j * ldb] : B[j + p * ldb], e, s, dB + j * k + p, n * k);
+  }
+  return 0;
+}
+
+ORC_API int orc_dgemm_ozaki_i8(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+                               const double* A, int64_t lda, const double* B, int64_t ldb,
+                               double beta, double* C, int64_t ldc, int s, int nthreads) {
+  if (s < 1 || s > 8) return -16;
+  if (alpha == 0.0 || k == 0) {
+    for (int64_t j = 0; j < n; ++j)
+      for (int64_t i = 0; i < m; ++i) C[i + j * ldc] = beta == 0.0 ? 0.0 : beta * C[i + j * ldc];
+    return 0;
+  }
+  int* eA = (int*)malloc(sizeof(int) * (m + 1));
+  int* eB = (int*)malloc(sizeof(int) * (n + 1));
+  int16_t* dA = (int16_t*)malloc(sizeof(int16_t) * (size_t)(s * m * k + 1));
+  int16_t* dB = (int16_t*)malloc(sizeof(int16_t) * (size_t)(s * n * k + 1));
+  int rc = 0;
+  if (!eA || !eB || !dA || !dB) { rc = 2; goto done; }
+  if (orc_ozaki_i8_split_a(ta, m, k, A, lda, s, eA, dA) || orc_ozaki_i8_split_b(tb, k, n, B, ldb, s, eB, dB)) {
+    rc = 1;
+    goto done;
+  }
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t j = 0; j < n; ++j)
+    for (int64_t i = 0; i < m; ++i) {
+      double acc = ozaki_entry(m, n, k, s, ORC_I8_CHUNK, eA, dA, eB, dB, i, j);
       double c = alpha * acc;
       if (beta != 0.0) c = fma(beta, C[i + j * ldc], c);
       C[i + j * ldc] = c;

@@ -65,10 +65,16 @@ def lib():
     L.orc_dgemm_ozaki.argtypes = [C.c_char, C.c_char, i64, i64, i64, dbl, p, i64, p, i64, dbl, p, i64, C.c_int, C.c_int]
     L.orc_ozaki_chunk.argtypes = [C.c_int]
     L.orc_ozaki_chunk.restype = C.c_int
+    L.orc_ozaki_i8_split_a.argtypes = [C.c_char, i64, i64, p, i64, C.c_int, p, p]
+    L.orc_ozaki_i8_split_b.argtypes = [C.c_char, i64, i64, p, i64, C.c_int, p, p]
+    L.orc_dgemm_ozaki_i8.argtypes = [C.c_char, C.c_char, i64, i64, i64, dbl, p, i64, p, i64, dbl, p, i64, C.c_int,
+                                     C.c_int]
+    L.orc_ozaki_i8_chunk.argtypes = []
+    L.orc_ozaki_i8_chunk.restype = C.c_int
     L.orc_pick_kernel_size.argtypes = [i64, i64, i64, p]
     L.orc_blocking_structure.argtypes = [i64, i64, i64, i64, p, i64, i64, p]
     for fn in ("orc_jmul_d", "orc_jmul_s", "orc_dgemm_ozaki", "orc_ozaki_split_a", "orc_ozaki_split_b",
-               "orc_blocking_structure"):
+               "orc_dgemm_ozaki_i8", "orc_ozaki_i8_split_a", "orc_ozaki_i8_split_b", "orc_blocking_structure"):
         getattr(L, fn).restype = C.c_int
     _LIB = L
     return L
@@ -243,6 +249,44 @@ def dgemm_ozaki(ta, tb, alpha, A, B, beta, C, s, nthreads=None) -> np.ndarray:
 def ozaki_chunk(s: int) -> int:
     """k-chunk length over which the emulated path's FP32 level sums stay exact."""
     return int(lib().orc_ozaki_chunk(s))
+
+
+def ozaki_i8_split_a(ta, A, s):
+    """INT8-slice split of op(A)'s rows: exponents (m,), digits (s, m, k) in [-128, 127]."""
+    A = _f(A)
+    m, k = (A.shape if ta == "N" else A.shape[::-1])
+    e = np.zeros(m, dtype=np.int32)
+    d = np.zeros((s, m, k), dtype=np.int16)
+    if lib().orc_ozaki_i8_split_a(ta.encode(), m, k, _ptr(A), _ld(A), s, _ptr(e), _ptr(d)):
+        raise ValueError("non-finite input")
+    return e, d
+
+
+def ozaki_i8_split_b(tb, B, s):
+    B = _f(B)
+    k, n = (B.shape if tb == "N" else B.shape[::-1])
+    e = np.zeros(n, dtype=np.int32)
+    d = np.zeros((s, n, k), dtype=np.int16)
+    if lib().orc_ozaki_i8_split_b(tb.encode(), k, n, _ptr(B), _ld(B), s, _ptr(e), _ptr(d)):
+        raise ValueError("non-finite input")
+    return e, d
+
+
+def dgemm_ozaki_i8(ta, tb, alpha, A, B, beta, C, s, nthreads=None) -> np.ndarray:
+    """Exact restatement of the split-INT8 emulated DGEMM with s slices."""
+    A, B = _f(A), _f(B)
+    C = np.array(C, dtype=np.float64, order="F", copy=True)
+    m, n, k = _dims(ta, tb, A, B)
+    rc = lib().orc_dgemm_ozaki_i8(ta.encode(), tb.encode(), m, n, k, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B), beta,
+                                  _ptr(C), _ld(C), s, nthreads or nthreads_default())
+    if rc:
+        raise ValueError(f"orc_dgemm_ozaki_i8 rc={rc}")
+    return C
+
+
+def ozaki_i8_chunk() -> int:
+    """k-chunk length over which the INT8 path's int32 level sums stay exact."""
+    return int(lib().orc_ozaki_i8_chunk())
 
 
 def pick_kernel_size(register_size: int, register_count: int, t_size: int):

@@ -323,6 +323,99 @@ def test_ozaki_ops_alpha_beta_consistent():
         O.dgemm_ozaki("N", "N", 1.0, np.array([[np.inf]]), np.array([[1.0]]), 0.0, np.zeros((1, 1)), 3)
 
 
+def _i8_rounded(a, e, s):
+    """rint(a * 2^(8s-1-e)) as an exact Python integer (round half to even)."""
+    q = Fraction(float(a)) * Fraction(2) ** (8 * s - 1 - int(e))
+    fl = q.numerator // q.denominator
+    r = q - fl
+    if r > Fraction(1, 2) or (r == Fraction(1, 2) and fl % 2 == 1):
+        fl += 1
+    return fl
+
+
+def test_ozaki_i8_digits_are_signed_bytes_of_the_rounded_value():
+    # rows whose max sits just below a power of two (the top digit would be 127.x),
+    # exactly at one, tiny entries, mixed signs
+    rows = np.array([[0.99999, -0.3, 1e-9, 0.5],
+                     [-0.9883, 0.25, -2.0 ** -40, 0.1],
+                     [0.98828125, -0.98828125, 0.0, 3e-5],
+                     [1.0, -1.0, 0.75, -0.5],
+                     [-7.9, 3.3, 1e-300, 6.0]])
+    A = np.vstack([rows, G.matrix(41, 12, 4, kind="wide")])
+    for s in (1, 2, 4, 7, 8):
+        e, d = O.ozaki_i8_split_a("N", A, s)
+        assert d.min() >= -128 and d.max() <= 127
+        for i in range(A.shape[0]):
+            # the frexp exponent, bumped by one when max*2^-E > 126.5/128
+            f, ex = math.frexp(float(np.max(np.abs(A[i]))))
+            assert e[i] == (ex + 1 if f > 126.5 / 128 else ex)
+            for p in range(A.shape[1]):
+                X = sum(int(d[t, i, p]) * 256 ** (s - 1 - t) for t in range(s))
+                assert X == _i8_rounded(A[i, p], e[i], s), (s, i, p)
+    # op('T') splits the columns of A^T the same way
+    e1, d1 = O.ozaki_i8_split_a("N", A, 5)
+    e2, d2 = O.ozaki_i8_split_a("T", np.asfortranarray(A.T), 5)
+    assert np.array_equal(e1, e2) and np.array_equal(d1, d2)
+
+
+def test_ozaki_i8_exact_when_representable():
+    rng = np.random.default_rng(6)
+    m, n, k = 10, 11, 70
+    A = rng.integers(-100, 101, (m, k)) * 2.0 ** -7
+    B = rng.integers(-100, 101, (k, n)) * 2.0 ** -10
+    for s in (1, 2, 3):
+        got = O.dgemm_ozaki_i8("N", "N", 1.0, A, B, 0.0, np.zeros((m, n)), s)
+        assert np.array_equal(got, A @ B)
+
+
+def test_ozaki_i8_error_shrinks_per_slice_and_matches_bf16_split():
+    A = G.matrix(42, 48, 256, kind="wide")
+    B = G.matrix(43, 256, 40, kind="wide")
+    ref = O.gemm_dot2("N", "N", 1.0, A, B, 0.0, np.zeros((48, 40)))
+    scale = np.abs(A) @ np.abs(B)
+    errs, errs_bf = [], []
+    for s in range(2, 9):
+        got = O.dgemm_ozaki_i8("N", "N", 1.0, A, B, 0.0, np.zeros((48, 40)), s)
+        errs.append(np.max(np.abs(got - ref) / scale))
+        bf = O.dgemm_ozaki("N", "N", 1.0, A, B, 0.0, np.zeros((48, 40)), s)
+        errs_bf.append(np.max(np.abs(bf - ref) / scale))
+    for a, b in zip(errs, errs[1:]):
+        assert b < a * 2.0 ** -5
+    # same digit weights as the BF16 split: the same accuracy within a factor of 4
+    for s, e, eb in zip(range(2, 9), errs, errs_bf):
+        assert e < max(2.0 ** (-8 * s + 11), 2.0 ** -50)
+        assert e <= 4 * max(eb, 2.0 ** -52), (s, e, eb)
+
+
+def test_ozaki_i8_chunked_level_sums():
+    # k above one int32 chunk (8192): chunks flush into the FP64 groups; with
+    # one-digit inputs every partial sum is an integer below 2^53: A@B exactly
+    assert O.ozaki_i8_chunk() == 8192
+    rng = np.random.default_rng(8)
+    m, n, k = 3, 2, 20000
+    A = rng.integers(-126, 127, (m, k)) * 2.0 ** -7
+    B = rng.integers(-126, 127, (k, n)) * 2.0 ** -7
+    A[:, 0] = 126 / 128  # row/column max below 126.5/128: exponent 0, one exact digit
+    B[0, :] = 126 / 128
+    got = O.dgemm_ozaki_i8("N", "N", 1.0, A, B, 0.0, np.zeros((m, n)), 1)
+    ref = (A * 2 ** 7).astype(np.int64) @ (B * 2 ** 7).astype(np.int64)
+    assert np.array_equal(got, ref * 2.0 ** -14)
+
+
+def test_ozaki_i8_ops_alpha_beta_consistent():
+    A = G.matrix(44, 20, 33)
+    B = G.matrix(45, 33, 14)
+    C0 = G.matrix(46, 20, 14)
+    base = O.dgemm_ozaki_i8("N", "N", 1.5, A, B, 0.5, C0, 5)
+    for ta in "NT":
+        for tb in "NT":
+            Aop = A if ta == "N" else np.asfortranarray(A.T)
+            Bop = B if tb == "N" else np.asfortranarray(B.T)
+            assert np.array_equal(O.dgemm_ozaki_i8(ta, tb, 1.5, Aop, Bop, 0.5, C0, 5), base)
+    with pytest.raises(ValueError):
+        O.dgemm_ozaki_i8("N", "N", 1.0, np.array([[np.nan]]), np.array([[1.0]]), 0.0, np.zeros((1, 1)), 3)
+
+
 # ---------------------------------------------------------------------------
 # Tile policy (src/kernel_structure.jl:76-99, src/memory_management.jl:78-177)
 # ---------------------------------------------------------------------------
