@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--nslices", type=int, default=7)
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline and the oracle check")
     ap.add_argument("--skip-table", action="store_true", help="omit the per-slice-count table")
-    ap.add_argument("--only", choices=["ozaki", "native"], default=None,
+    ap.add_argument("--only", choices=["ozaki", "native", "ozaki_int8"], default=None,
                     help="run only one method (for profiler runs)")
     ap.add_argument("--skip-grouped", action="store_true",
                     help="omit the batched (1024 uneven problems) and rrule (4096x2048x4096) measurements")
@@ -127,6 +127,17 @@ def grouped_and_rrule(torch, jb):
                       "total_tflops": round(3 * fl / (ms_f + ms_b) / 1e9, 3)}
     out["rrule_4096x2048x4096_f32"] = rr
     return out, samples
+
+
+def profiled_traffic(key):
+    """HBM bytes per launch of a kernel at this workload, as measured by the
+    counter passes recorded in profiles/traffic.json (None if not profiled)."""
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(tpath) as f:
+            return json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
 
 
 def cpu_baseline_leg(A_sub_rows, B_sub_cols, A_panel, B_panel, nthreads, grouped_samples=None):
@@ -219,6 +230,9 @@ def main():
     def step_native():
         jb.gemm_("N", "N", 1.0, Ac, Bc, 0.0, Cc, method="native")
 
+    def step_int8():
+        jb.gemm_("N", "N", 1.0, Ac, Bc, 0.0, Cc, method="ozaki_int8", nslices=s)
+
     result = {}
     # ---- emulated DGEMM (the measured step) ----
     if args.only in (None, "ozaki"):
@@ -248,18 +262,35 @@ def main():
         result["native"] = {"ms": ms_n, "tflops": flops / ms_n / 1e9, "kernel_ms": kn_ms / max(kn_n, 1)}
         log(f"native: {ms_n:.3f} ms/step  {flops / ms_n / 1e9:.2f} TF")
         C_nat = C.clone() if not args.skip_cpu else None
+    # ---- emulated DGEMM on INT8 slices (same digit weights, int8 MFMA) ----
+    if args.only in (None, "ozaki_int8"):
+        for _ in range(args.warmup):
+            step_int8()
+        _lib.kernel_timing_reset()
+        _lib.kernel_timing(True)
+        ms_i = timed(torch, step_int8, args.steps)
+        _lib.kernel_timing(False)
+        ki_ms, ki_n = _lib.kernel_time("slice_pair_gemm_i8")
+        si_ms, si_n = _lib.kernel_time("ozaki_i8_split")
+        result["int8"] = {"ms": ms_i, "tflops": flops / ms_i / 1e9, "gemm_kernel_ms": ki_ms / max(ki_n, 1),
+                          "split_ms": si_ms / max(si_n, 1)}
+        log(f"ozaki_int8 s={s}: {ms_i:.3f} ms/step  {flops / ms_i / 1e9:.2f} TF  (slice-pair kernel "
+            f"{ki_ms / max(ki_n, 1):.3f} ms)")
+        C_i8 = C.clone() if not args.skip_cpu else None
 
     # ---- slice-count table (after the timed region) ----
     table = []
     if args.only is None and not args.skip_table:
-        for ss in range(3, 9):
-            def f(ss=ss):
-                jb.gemm_("N", "N", 1.0, Ac, Bc, 0.0, Cc, method="ozaki", nslices=ss)
-            f()
-            ms_s = timed(torch, f, 2)
-            table.append({"nslices": ss, "ms": round(ms_s, 3), "tflops": round(flops / ms_s / 1e9, 2),
-                          "out": C.clone() if not args.skip_cpu else None})
-            log(f"  table s={ss}: {ms_s:.3f} ms  {flops / ms_s / 1e9:.2f} TF")
+        for meth, srange in (("ozaki", range(3, 9)), ("ozaki_int8", range(5, 9))):
+            for ss in srange:
+                def f(ss=ss, meth=meth):
+                    jb.gemm_("N", "N", 1.0, Ac, Bc, 0.0, Cc, method=meth, nslices=ss)
+                f()
+                ms_s = timed(torch, f, 2)
+                table.append({"method": meth, "nslices": ss, "ms": round(ms_s, 3),
+                              "tflops": round(flops / ms_s / 1e9, 2),
+                              "out": C.clone() if not args.skip_cpu else None})
+                log(f"  table {meth} s={ss}: {ms_s:.3f} ms  {flops / ms_s / 1e9:.2f} TF")
 
     # ---- batched and rrule workloads (after the timed region) ----
     extra, gsamples = None, None
@@ -296,6 +327,8 @@ def main():
             errs["ozaki"] = err(C_oz)
         if "native" in result:
             errs["native"] = err(C_nat)
+        if "int8" in result:
+            errs["int8"] = err(C_i8)
         for row in table:
             row["max_rel_err"], row["max_err_over_absAB"] = err(row.pop("out"))
             if "native" in errs:
@@ -311,7 +344,8 @@ def main():
 
     oz = result.get("ozaki")
     nat = result.get("native")
-    main_ms = oz["ms"] if oz else nat["ms"]
+    i8 = result.get("int8")
+    main_ms = oz["ms"] if oz else (nat["ms"] if nat else i8["ms"])
     line = {
         "metric": METRIC,
         "value": round(flops / main_ms / 1e9, 3),
@@ -323,7 +357,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16" if oz else "f64",
+        "dtype": "bf16" if oz else ("f64" if nat else "int8"),
         "data": "synthetic (seeded SplitMix64 in HBM: a_ij = u*2^e, u in [-1,1), e in [-12,12])",
         "config": {"workload": f"dgemm {n}x{n}x{n} column-major NN, split-BF16 emulated with {s} slices "
                                f"({pairs} BF16 slice-pair products, FP32 exact level sums, FP64 combine) "
@@ -337,6 +371,21 @@ def main():
         line["native_fp64_frac_of_peak"] = round(nat["tflops"] / info["peak_fp64_tflops"], 4)
     if oz and nat:
         line["speedup_emulated_vs_native"] = round(nat["ms"] / oz["ms"], 4)
+    if i8:
+        line["emulated_int8"] = {"method": f"ozaki_int8, {s} INT8 slices on v_mfma_i32_32x32x32_i8 (exact int32 "
+                                           f"level sums), same digit weights and FP64 combine",
+                                 "tflops": round(i8["tflops"], 3), "ms_per_step": round(i8["ms"], 4),
+                                 "split_kernels_ms": round(i8["split_ms"], 4)}
+        if nat:
+            line["emulated_int8"]["speedup_vs_native"] = round(nat["ms"] / i8["ms"], 4)
+        hw = pairs * flops
+        ach_i = hw / (i8["gemm_kernel_ms"] * 1e-3) / 1e12
+        peak_i8 = info["peak_bf16_tflops"] * 2.0  # int8 MFMA: 8192 ops/clk/CU
+        line["emulated_int8"]["roofline"] = {"kernel": f"slice_pair_gemm_kernel<2,{s},{s + 1},1>", "bound": "mfma",
+                                             "achieved": round(ach_i, 2), "peak": round(peak_i8, 1),
+                                             "unit": "TOP/s", "frac": round(ach_i / peak_i8, 4),
+                                             "traffic": profiled_traffic(f"slice_pair_gemm_i8_s{s}_n{n}"),
+                                             "avg_launch_ms": round(i8["gemm_kernel_ms"], 4)}
     if errs:
         if "ozaki" in errs:
             line["max_rel_err_emulated"] = errs["ozaki"][0]
@@ -344,19 +393,16 @@ def main():
             line["max_rel_err_native"] = errs["native"][0]
         if "ozaki" in errs and "native" in errs:
             line["err_ratio_emulated_vs_native"] = round(errs["ozaki"][0] / errs["native"][0], 4)
+        if "int8" in errs and "emulated_int8" in line:
+            line["emulated_int8"]["max_rel_err"] = errs["int8"][0]
+            if "native" in errs:
+                line["emulated_int8"]["err_ratio_vs_native"] = round(errs["int8"][0] / errs["native"][0], 4)
         line["error_sample"] = "Dot2 oracle on 64 sampled rows x 64 sampled columns (4096 entries)"
     # roofline of the dominant kernel
     if oz:
         hw_flops = pairs * flops
         ach = hw_flops / (oz["gemm_kernel_ms"] * 1e-3) / 1e12
-        traffic = None
-        tpath = os.path.join(ROOT, "profiles", "traffic.json")
-        if os.path.exists(tpath):
-            try:
-                with open(tpath) as f:
-                    traffic = json.load(f).get(f"slice_pair_gemm_s{s}_n{n}")
-            except Exception:
-                traffic = None
+        traffic = profiled_traffic(f"slice_pair_gemm_s{s}_n{n}")
         line["roofline"] = {"kernel": f"slice_pair_gemm_kernel<0,{s},{s + 1},1>", "bound": "mfma",
                             "achieved": round(ach, 2), "peak": round(info["peak_bf16_tflops"], 1),
                             "unit": "TFLOP/s", "frac": round(ach / info["peak_bf16_tflops"], 4),
