@@ -1,4 +1,4 @@
-// k_ozaki.hip -- split-BF16 emulated DGEMM and 3xBF16 SGEMM on gfx950.
+// k_ozaki.hip -- split-BF16 and split-INT8 emulated DGEMM and 3xBF16 SGEMM on gfx950.
 //
 // The reference has no emulated path (its GEMM is the FP64/FP32 register tile
 // of src/gemm.jl:244-348); this file implements the method of the emulated
@@ -13,11 +13,13 @@
 //      64 for op(B)) and a k-stage of 16, the RB x 16 digits form one
 //      contiguous block (row-major, 32 bytes per row), so a GEMM stage is a
 //      handful of contiguous 1 KB pieces per plane.
-//   3. slice-pair GEMM (MFMA-bound): 4 waves per block, one per SIMD, each
-//      owning a 64 x 32 tile (two 32x32 MFMA blocks), one FP32 accumulator set
-//      per level l = t+u (pairs t+u <= s+1) on v_mfma_f32_32x32x16_bf16.
-//      Operand stages stream into a 3-slot LDS ring by direct global->LDS
-//      loads (global_load_lds_dwordx4), issued two stages ahead, one
+//   3. slice-pair GEMM (MFMA-bound): 8 waves per block (2 per SIMD), each
+//      owning a 32 x 32 tile of the 128 x 64 block tile, one FP32 accumulator
+//      set per level l = t+u (pairs t+u <= s+1) on v_mfma_f32_32x32x16_bf16,
+//      pairs in square order (step j: the new slices A_j, B_j against all
+//      earlier ones, so a step needs two new fragments).  Operand stages stream
+//      into a 3-slot LDS ring by direct global->LDS loads
+//      (global_load_lds_dwordx4), two stages ahead, one piece per step, one
 //      s_barrier per stage.  Every kc k-values (kc*s <= 1024, so a level sum
 //      stays below 2^24 and is exact whatever the MFMA's internal order) each
 //      level is flushed into an FP64 group accumulator right after its last
@@ -26,6 +28,10 @@
 //      TwoSum from the smallest, rescales by 2^(E_i+F_j-14) and applies
 //      alpha/beta.  These are the oracle restatement's operations in the same
 //      order, so the result is bit-reproducible.
+//   INT8 mode: the same digit weights with signed-byte digits (split 2c), the
+//   same LDS image (a stage of 32 int8 k has the bytes of a BF16 stage of 16),
+//   v_mfma_i32_32x32x32_i8 with exact int32 level sums over k-chunks of 8192,
+//   the same flush and combine.
 //   BF16X3 mode (FP32 inputs): per-element split a = a1 + a2 + a3 (bf16 round
 //   to nearest of the running residual, exact), products t+u <= LMAX, level 2
 //   in one FP32 accumulator and the smaller levels in a second.
