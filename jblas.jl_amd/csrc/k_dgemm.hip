@@ -30,6 +30,10 @@ template <> struct Vec16<float> { typedef float type __attribute__((ext_vector_t
 constexpr int NT = 256;  // threads per block (4 waves)
 constexpr int BK = 16;   // k extent of one LDS stage
 
+// Column XOR of k-row k in a swizzled k-major LDS tile: flips bits 2-4 of the
+// row index, so a 16-row fragment read stays 16 consecutive elements.
+__device__ __forceinline__ int kswz(int k) { return 4 * ((k >> 1) & 7); }
+
 // Loads one R x BK operand tile (R rows of op(A) or columns of op(B)) into
 // registers, then writes it k-major into LDS.
 //   MN_CONTIG: element (r, kk) at P[(r0+r) + (k0+kk)*ld]  (A 'N', B 'T')
@@ -84,7 +88,10 @@ struct TileLoader {
     }
   }
 
-  template <int STRIDE>
+  // SWZ (k-contiguous loads only): row r of k-row k is stored at column
+  // r ^ kswz(k), which spreads the 8 k-rows one store instruction writes
+  // over distinct banks; readers apply the same XOR.
+  template <int STRIDE, bool SWZ = false>
   __device__ __forceinline__ void store(T* S) const {
     const int t = threadIdx.x;
     if (MN_CONTIG) {
@@ -99,7 +106,7 @@ struct TileLoader {
       for (int i = 0; i < PER; ++i) {
         const int r = rb + (NT / KV) * i;
 #pragma unroll
-        for (int e = 0; e < VW; ++e) S[(kk + e) * STRIDE + r] = v[i][e];
+        for (int e = 0; e < VW; ++e) S[(kk + e) * STRIDE + (SWZ ? (r ^ kswz(kk + e)) : r)] = v[i][e];
       }
     }
   }
@@ -137,11 +144,14 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
                                                           const T* __restrict__ A, int64_t lda,
                                                           const T* __restrict__ B, int64_t ldb, T beta,
                                                           T* __restrict__ C, int64_t ldc, int tiles_m, int tiles_n) {
-  // LDS row stride in elements: MB plus DG_PAD_BYTES of padding (32 B keeps
-  // rows 16-byte aligned for the vector stores; measured faster on the
-  // transposed-A loader than a 128-byte pad that separates the lane groups'
-  // banks).
-  constexpr int STRIDE = MB + DG_PAD_BYTES / (int)sizeof(T);
+  // LDS row stride in elements.  FP64: 144 = MB + 16, so the k-rows read by
+  // lane groups g and g+1 of a ds_read_b64 start 32 banks apart (no bank
+  // conflicts); the k-contiguous operands (A 'T', B 'N') are stored with the
+  // row XOR kswz(k), without which their 8-k-row stores would hit the same
+  // banks at this stride.  FP32: MB plus DG_PAD_BYTES of padding.
+  constexpr bool F64 = sizeof(T) == 8;
+  constexpr int STRIDE = F64 ? MB + 16 : MB + DG_PAD_BYTES / (int)sizeof(T);
+  constexpr bool SWA = F64 && TA, SWB = F64 && !TB;
   __shared__ __attribute__((aligned(16))) T As[2][BK * STRIDE];
   __shared__ __attribute__((aligned(16))) T Bs[2][BK * STRIDE];
   typedef typename Mfma<T>::acc_t acc_t;
@@ -169,8 +179,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
   const int nkt = (int)((K + BK - 1) / BK);
   la.load(A, lda, m0, 0, M, K, fullA && BK <= K);
   lb.load(B, ldb, n0, 0, N, K, fullB && BK <= K);
-  la.template store<STRIDE>(As[0]);
-  lb.template store<STRIDE>(Bs[0]);
+  la.template store<STRIDE, SWA>(As[0]);
+  lb.template store<STRIDE, SWB>(Bs[0]);
   __syncthreads();
 
   for (int kt = 0; kt < nkt; ++kt) {
@@ -188,9 +198,11 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
       // MFMA t covers k = 4t..4t+3; lane group g supplies k = 4t+g.
       T a[4], b[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = as[(4 * t + g) * STRIDE + wm0 + 16 * i + l16];
+      for (int i = 0; i < 4; ++i)
+        a[i] = as[(4 * t + g) * STRIDE + ((wm0 + 16 * i + l16) ^ (SWA ? kswz(4 * t + g) : 0))];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = bs[(4 * t + g) * STRIDE + wn0 + 16 * j + l16];
+      for (int j = 0; j < 4; ++j)
+        b[j] = bs[(4 * t + g) * STRIDE + ((wn0 + 16 * j + l16) ^ (SWB ? kswz(4 * t + g) : 0))];
       // D'[p][q] = sum_k X[p][k] Y[k][q] with X = op(B)^T block (p -> n) and Y = op(A)^T
       // block (q -> m), so a 16-lane group holds 16 consecutive rows of one column of C
       // (coalesced column-major stores).  The f64 MFMA is a sequential fma chain over
@@ -202,8 +214,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
         for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::mma(b[j], a[i], acc[i][j]);
     }
     if (kt + 1 < nkt) {
-      la.template store<STRIDE>(As[cur ^ 1]);
-      lb.template store<STRIDE>(Bs[cur ^ 1]);
+      la.template store<STRIDE, SWA>(As[cur ^ 1]);
+      lb.template store<STRIDE, SWB>(Bs[cur ^ 1]);
     }
     __syncthreads();
   }
