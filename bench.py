@@ -129,6 +129,70 @@ def grouped_and_rrule(torch, jb):
     return out, samples
 
 
+def native_and_boundary(torch, jb):
+    """Secondary lines, after the timed region:
+    * native FP64 MFMA DGEMM, C = 1.5*op(A)*op(B) + 0.5*C, M=N=K in {1024,
+      4096, 8192}, op in {N,T}^2 (device-resident operands);
+    * jmul! at 256^3 (the reference's CPU-runnable case): device-resident
+      call and the host-pointer call (operands staged over PCIe);
+    * the host-pointer DGEMM at 8192^3 (PCIe-inclusive: A and B copied in,
+      C copied out), native and split-BF16 s = 7."""
+    import numpy as np
+
+    out = {"c2_native_tflops": {}}
+    for n in (1024, 4096, 8192):
+        A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        B = torch.empty_like(A)
+        C = torch.empty_like(A)
+        jb.fill_uniform_(A, 42)
+        jb.fill_uniform_(B, 43)
+        jb.fill_uniform_(C, 44)
+        for ta in "NT":
+            for tb in "NT":
+                def f(ta=ta, tb=tb):
+                    jb.gemm_(ta, tb, 1.5, A.T, B.T, 0.5, C.T, method="native")
+                f()
+                reps = 20 if n < 8192 else 5
+                ms = timed(torch, f, reps)
+                out["c2_native_tflops"][f"{ta}{tb}_{n}"] = round(2.0 * n ** 3 / ms / 1e9, 2)
+        del A, B, C
+        torch.cuda.empty_cache()
+    # jmul! 256^3: device-resident and host-pointer
+    n = 256
+    Ad = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    Xd = torch.empty_like(Ad)
+    Dd = torch.empty_like(Ad)
+    jb.fill_uniform_(Ad, 42)
+    jb.fill_uniform_(Xd, 43)
+    jb.jmul_(Dd.T, Ad.T, Xd.T)
+    ms_dev = timed(torch, lambda: jb.jmul_(Dd.T, Ad.T, Xd.T), 200)
+    Ah, Xh = np.asfortranarray(Ad.cpu().numpy().T), np.asfortranarray(Xd.cpu().numpy().T)
+    Dh = np.zeros((n, n), order="F")
+    jb.jmul_(Dh, Ah, Xh)
+    t0 = time.perf_counter()
+    for _ in range(50):
+        jb.jmul_(Dh, Ah, Xh)
+    ms_host = (time.perf_counter() - t0) / 50 * 1e3
+    out["c1_jmul_256"] = {"device_us": round(ms_dev * 1e3, 2), "device_gflops": round(2 * n ** 3 / ms_dev / 1e6, 2),
+                          "host_pointer_us": round(ms_host * 1e3, 2),
+                          "host_pointer_gflops": round(2 * n ** 3 / ms_host / 1e6, 2)}
+    # host-pointer DGEMM at 8192^3 (PCIe-inclusive)
+    n = 8192
+    Ah = np.asfortranarray(np.random.default_rng(1).uniform(-1, 1, (n, n)))
+    Bh = np.asfortranarray(np.random.default_rng(2).uniform(-1, 1, (n, n)))
+    Ch = np.zeros((n, n), order="F")
+    host = {}
+    for meth, kw in (("native", {}), ("ozaki_s7", {"method": "ozaki", "nslices": 7})):
+        jb.gemm_("N", "N", 1.0, Ah, Bh, 0.0, Ch, **kw)
+        t0 = time.perf_counter()
+        for _ in range(2):
+            jb.gemm_("N", "N", 1.0, Ah, Bh, 0.0, Ch, **kw)
+        ms = (time.perf_counter() - t0) / 2 * 1e3
+        host[meth] = {"ms": round(ms, 2), "tflops": round(2.0 * n ** 3 / ms / 1e9, 2)}
+    out["host_pointer_8192_pcie_inclusive"] = host
+    return out
+
+
 def profiled_traffic(key):
     """HBM bytes per launch of a kernel at this workload, as measured by the
     counter passes recorded in profiles/traffic.json (None if not profiled)."""
@@ -297,6 +361,10 @@ def main():
     if args.only is None and not args.skip_grouped:
         extra, gsamples = grouped_and_rrule(torch, jb)
         log(f"grouped/rrule: {extra}")
+    secondary = None
+    if args.only is None and not args.skip_table:
+        secondary = native_and_boundary(torch, jb)
+        log(f"native table / boundary: {secondary}")
 
     # ---- CPU baseline + checker (rank 0, N = 1) ----
     cpu = None
@@ -423,6 +491,8 @@ def main():
         line["nslices_table"] = table
     if extra:
         line["batched_and_rrule"] = extra
+    if secondary:
+        line.update(secondary)
     if cpu:
         line["cpu_baseline"] = {"value": round(cpu["mt_tflops"], 5), "unit": "TFLOP/s", "cores": cpu["mt_threads"],
                                 "kind": "port",

@@ -998,11 +998,15 @@ int jb_gemm_pullback(int method, char transA, char transB, int64_t m, int64_t n,
   hipError_t e = hipSuccess;
   auto h2d = [&](void* dst, const void* src, int64_t rows, int64_t cols, int64_t ld) {
     if (e == hipSuccess && rows > 0 && cols > 0)
-      e = hipMemcpy2DAsync(dst, rows * elt, src, ld * elt, rows * elt, cols, hipMemcpyHostToDevice, s);
+      e = (ld == rows || cols == 1)
+              ? hipMemcpyAsync(dst, src, (size_t)rows * cols * elt, hipMemcpyHostToDevice, s)
+              : hipMemcpy2DAsync(dst, rows * elt, src, ld * elt, rows * elt, cols, hipMemcpyHostToDevice, s);
   };
   auto d2h = [&](void* dst, const void* src, int64_t rows, int64_t cols, int64_t ld) {
     if (e == hipSuccess && rows > 0 && cols > 0)
-      e = hipMemcpy2DAsync(dst, ld * elt, src, rows * elt, rows * elt, cols, hipMemcpyDeviceToHost, s);
+      e = (ld == rows || cols == 1)
+              ? hipMemcpyAsync(dst, src, (size_t)rows * cols * elt, hipMemcpyDeviceToHost, s)
+              : hipMemcpy2DAsync(dst, ld * elt, src, rows * elt, rows * elt, cols, hipMemcpyDeviceToHost, s);
   };
   if (dB) h2d(dAv, A, ar, ac, lda);
   if (dA) h2d(dBv, B, br, bc, ldb);

@@ -173,12 +173,18 @@ int ws_release(hipStream_t s) {
 // ---- host staging ----------------------------------------------------------
 // Copies a rows x cols column-major host matrix (leading dim ld) to device
 // memory with the same leading dimension.
+// A dense matrix (ld == rows) is one contiguous block: one 1-D copy (a 2-D copy
+// of pageable memory is carried out row by row).
 static hipError_t h2d(void* dst, const void* src, int64_t rows, int64_t cols, int64_t ld, size_t elt, hipStream_t s) {
   if (rows == 0 || cols == 0) return hipSuccess;
+  if (ld == rows || cols == 1)
+    return hipMemcpyAsync(dst, src, (size_t)(ld * (cols - 1) + rows) * elt, hipMemcpyHostToDevice, s);
   return hipMemcpy2DAsync(dst, ld * elt, src, ld * elt, rows * elt, cols, hipMemcpyHostToDevice, s);
 }
 static hipError_t d2h(void* dst, const void* src, int64_t rows, int64_t cols, int64_t ld, size_t elt, hipStream_t s) {
   if (rows == 0 || cols == 0) return hipSuccess;
+  if (ld == rows || cols == 1)
+    return hipMemcpyAsync(dst, src, (size_t)(ld * (cols - 1) + rows) * elt, hipMemcpyDeviceToHost, s);
   return hipMemcpy2DAsync(dst, ld * elt, src, ld * elt, rows * elt, cols, hipMemcpyDeviceToHost, s);
 }
 static size_t span_bytes(int64_t rows, int64_t cols, int64_t ld, size_t elt) {

@@ -114,10 +114,11 @@ struct TileLoader {
 
 // ---------------------------------------------------------------------------
 // (1) MFMA GEMM: 128x128 block tile, 4 waves of 64x64 (4x4 MFMA blocks of
-//     16x16), BK=16 per LDS stage, two LDS stages.  T = double uses
+//     16x16), or 64x64 with waves of 32x32 when the 128-tile grid would
+//     leave the device's last round mostly idle; BK=16 per LDS stage, two
+//     LDS stages.  T = double uses
 //     v_mfma_f64_16x16x4f64, T = float v_mfma_f32_16x16x4f32.
 // ---------------------------------------------------------------------------
-constexpr int MB = 128, NB = 128;
 #ifndef DG_PAD_BYTES
 #define DG_PAD_BYTES 32
 #endif
@@ -139,18 +140,21 @@ template <> struct Mfma<float> {
   static __device__ __forceinline__ int prow(int r, int g) { return 4 * g + r; }
 };
 
-template <typename T, bool TA, bool TB, bool VEC>
+// BT: square block tile, 128 (4 waves of 64 x 64) or 64 (4 waves of 32 x 32,
+// for problems with too few 128-tiles to fill the device).
+template <typename T, bool TA, bool TB, bool VEC, int BT>
 __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, int64_t K, T alpha,
                                                           const T* __restrict__ A, int64_t lda,
                                                           const T* __restrict__ B, int64_t ldb, T beta,
                                                           T* __restrict__ C, int64_t ldc, int tiles_m, int tiles_n) {
-  // LDS row stride in elements.  FP64: 144 = MB + 16, so the k-rows read by
+  // LDS row stride in elements.  FP64: BT + 16 (144 or 80), so the k-rows read by
   // lane groups g and g+1 of a ds_read_b64 start 32 banks apart (no bank
   // conflicts); the k-contiguous operands (A 'T', B 'N') are stored with the
   // row XOR kswz(k), without which their 8-k-row stores would hit the same
-  // banks at this stride.  FP32: MB plus DG_PAD_BYTES of padding.
+  // banks at this stride.  FP32: BT plus DG_PAD_BYTES of padding.
   constexpr bool F64 = sizeof(T) == 8;
-  constexpr int STRIDE = F64 ? MB + 16 : MB + DG_PAD_BYTES / (int)sizeof(T);
+  constexpr int STRIDE = F64 ? BT + 16 : BT + DG_PAD_BYTES / (int)sizeof(T);
+  constexpr int WI = BT / 32;  // 16 x 16 MFMA blocks per wave along m and n
   constexpr bool SWA = F64 && TA, SWB = F64 && !TB;
   __shared__ __attribute__((aligned(16))) T As[2][BK * STRIDE];
   __shared__ __attribute__((aligned(16))) T Bs[2][BK * STRIDE];
@@ -160,21 +164,21 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
   const int tile = xcd_remap(blockIdx.x, nblocks, 8);
   int tm, tn;
   tile_coords(tile, tiles_m, tiles_n, 8, tm, tn);
-  const int64_t m0 = (int64_t)tm * MB, n0 = (int64_t)tn * NB;
+  const int64_t m0 = (int64_t)tm * BT, n0 = (int64_t)tn * BT;
 
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-  const int wm0 = (wave % 2) * 64, wn0 = (wave / 2) * 64;
+  const int wm0 = (wave % 2) * (BT / 2), wn0 = (wave / 2) * (BT / 2);
   const int g = lane / 16, l16 = lane % 16;
 
-  TileLoader<T, MB, !TA, VEC> la;
-  TileLoader<T, NB, TB, VEC> lb;
-  const bool fullA = m0 + MB <= M, fullB = n0 + NB <= N;
+  TileLoader<T, BT, !TA, VEC> la;
+  TileLoader<T, BT, TB, VEC> lb;
+  const bool fullA = m0 + BT <= M, fullB = n0 + BT <= N;
 
-  acc_t acc[4][4];
+  acc_t acc[WI][WI];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < WI; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < WI; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   const int nkt = (int)((K + BK - 1) / BK);
   la.load(A, lda, m0, 0, M, K, fullA && BK <= K);
@@ -196,12 +200,12 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
 #pragma unroll
     for (int t = 0; t < BK / 4; ++t) {
       // MFMA t covers k = 4t..4t+3; lane group g supplies k = 4t+g.
-      T a[4], b[4];
+      T a[WI], b[WI];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < WI; ++i)
         a[i] = as[(4 * t + g) * STRIDE + ((wm0 + 16 * i + l16) ^ (SWA ? kswz(4 * t + g) : 0))];
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < WI; ++j)
         b[j] = bs[(4 * t + g) * STRIDE + ((wn0 + 16 * j + l16) ^ (SWB ? kswz(4 * t + g) : 0))];
       // D'[p][q] = sum_k X[p][k] Y[k][q] with X = op(B)^T block (p -> n) and Y = op(A)^T
       // block (q -> m), so a 16-lane group holds 16 consecutive rows of one column of C
@@ -209,9 +213,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
       // its four k values in lane-group order, so each dot product is one fma chain in
       // increasing k, as in the reference recurrence.
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < WI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::mma(b[j], a[i], acc[i][j]);
+        for (int j = 0; j < WI; ++j) acc[i][j] = Mfma<T>::mma(b[j], a[i], acc[i][j]);
     }
     if (kt + 1 < nkt) {
       la.template store<STRIDE, SWA>(As[cur ^ 1]);
@@ -221,10 +225,10 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
   }
 
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < WI; ++i) {
     const int64_t m = m0 + wm0 + 16 * i + l16;
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < WI; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t n = n0 + wn0 + 16 * j + Mfma<T>::prow(r, g);
@@ -328,9 +332,9 @@ bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 
 
 }  // namespace
 
-template <typename T>
-static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
-  const int tiles_m = (int)((g.m + MB - 1) / MB), tiles_n = (int)((g.n + NB - 1) / NB);
+template <typename T, int BT>
+static hipError_t launch_gemm_mfma_bt(const GemmArgs& g, hipStream_t s) {
+  const int tiles_m = (int)((g.m + BT - 1) / BT), tiles_n = (int)((g.n + BT - 1) / BT);
   const int nblocks = tiles_m * tiles_n;
   constexpr int VW = 16 / sizeof(T);
   const bool vec = aligned16(g.A) && aligned16(g.B) && (g.lda % VW == 0) && (g.ldb % VW == 0);
@@ -340,7 +344,7 @@ static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
   const T alpha = (T)g.alpha, beta = (T)g.beta;
   const char* tname = sizeof(T) == 8 ? "dgemm_mfma" : "sgemm_mfma";
 #define JB_LAUNCH(TA, TB, V)                                                                           \
-  hipLaunchKernelGGL((gemm_mfma_kernel<T, TA, TB, V>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, alpha, \
+  hipLaunchKernelGGL((gemm_mfma_kernel<T, TA, TB, V, BT>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, alpha, \
                      A, g.lda, B, g.ldb, beta, C, g.ldc, tiles_m, tiles_n)
   timing_begin(s, tname);
   if (vec) {
@@ -357,6 +361,19 @@ static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
 #undef JB_LAUNCH
   timing_end(s, tname);
   return hipGetLastError();
+}
+
+// 128-tiles run two blocks per CU (512 on the device at a time), 64-tiles at
+// least four and ~5 % below the 128-tile rate when both fill the device.  The
+// 128-tile grid is used when its last round of 512 is at least 90 % full
+// (measured at n = 1536..8192: 64-tiles win at 1536-3584 and 5120, 128-tiles
+// at 4096, 6144, 8192).
+template <typename T>
+static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
+  const int64_t tiles128 = ((g.m + 127) / 128) * ((g.n + 127) / 128);
+  const int64_t slots = ((tiles128 + 511) / 512) * 512;
+  return (double)tiles128 >= 0.9 * (double)slots ? launch_gemm_mfma_bt<T, 128>(g, s)
+                                                 : launch_gemm_mfma_bt<T, 64>(g, s);
 }
 
 hipError_t launch_dgemm_mfma(const GemmArgs& g, hipStream_t s) { return launch_gemm_mfma<double>(g, s); }
