@@ -20,11 +20,12 @@
 //      earlier ones, so a step needs two new fragments).  Operand stages stream
 //      into a 3-slot LDS ring by direct global->LDS loads
 //      (global_load_lds_dwordx4), two stages ahead, one piece per step, one
-//      s_barrier per stage.  Every kc k-values (kc*s <= 1024, so a level sum
-//      stays below 2^24 and is exact whatever the MFMA's internal order) each
-//      level is flushed into an FP64 group accumulator right after its last
-//      MFMA of the stage, G_g = fma(S_l, 2^-8j, G_g) with l = 2+4g+j, so the
-//      flush overlaps the remaining MFMAs.  The epilogue combines the groups by
+//      s_barrier per stage.  Every kc_l k-values (kc_l*(l+1) <= 1024 for the
+//      l+1 pairs of level l, so a level sum stays within 2^24 and is exact
+//      whatever the MFMA's internal order) level l is flushed into an FP64
+//      group accumulator right after its last MFMA of the stage,
+//      G_g = fma(S_l, 2^-8j, G_g) with l = 4g+j, so the flush overlaps the
+//      remaining MFMAs.  The epilogue combines the groups by
 //      TwoSum from the smallest, rescales by 2^(E_i+F_j-14) and applies
 //      alpha/beta.  These are the oracle restatement's operations in the same
 //      order, so the result is bit-reproducible.
@@ -64,9 +65,11 @@ constexpr int QPAD_K = 64;          // k padding of the planes (split tile width
 #define X3_WB 2  // 3xBF16: 4 waves of 64 x 32
 #endif
 
-__host__ __device__ constexpr int chunk_len(int s) {
+// k-chunk of 0-based level L (L + 1 slice pairs) over which its FP32 sum of
+// products of digits |d| <= 128 stays within 2^24, i.e. exact.
+__host__ __device__ constexpr int level_chunk(int L) {
   int kc = 1024;
-  while (kc * s > 1024) kc >>= 1;
+  while (kc * (L + 1) > 1024) kc >>= 1;
   return kc;
 }
 
@@ -344,9 +347,10 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   constexpr int WL_MAX = (NWL + NWAVES - 1) / NWAVES;  // ... of waves < NWL % NWAVES
   constexpr int NL = (MODE == 1) ? 2 : S;         // accumulator sets (levels)
   constexpr int NG = (S + 3) / 4;                 // FP64 groups (MODES 0, 2)
-  // stages per exact chunk: FP32 level sums below 2^24 (MODE 0); int32 sums of
-  // 8192 k (|S_l| <= 8 * 8192 * 2^14 = 2^30) in MODE 2, whose stages hold 32 k
-  constexpr int KC_ST = (MODE == 2) ? 8192 / 32 : chunk_len(S) / QBK;
+  // stages per exact chunk of level L: FP32 level sums within 2^24 (MODE 0:
+  // level_chunk(L) k); int32 sums of 8192 k (|S_l| <= 8 * 8192 * 2^14 = 2^30)
+  // in MODE 2, whose stages hold 32 k.  All powers of two.
+  auto kc_stages = [](int L) { return (MODE == 2) ? 8192 / 32 : level_chunk(L) / QBK; };
   typedef typename std::conditional<MODE == 2, i16v, f16v>::type AccV;
   __shared__ __attribute__((aligned(1024))) char ring[QSLOTS * STAGE_B];
 
@@ -436,7 +440,8 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
     const bool prefetch = st + 2 < nst;
     const int pslot = (st + 2) % QSLOTS;
     const u4* L = reinterpret_cast<const u4*>(ring + (st % QSLOTS) * STAGE_B);
-    const bool chunk_end = (MODE != 1) && ((((st + 1) % KC_ST) == 0) || (st + 1 == nst));
+    const int st1 = st + 1;  // a level's chunk ends after this stage if st1 is a multiple of its length
+    const bool last_stage = st1 == nst;
     // Pairs run in square order: step j multiplies the new slices A_j and B_j
     // with all earlier ones (pairs with max(t, u) == j).  A step needs only two
     // new fragments, read one step ahead, so a stage starts after two LDS reads
@@ -474,7 +479,7 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
           else
             acc[l][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bf[u], af[t][b], acc[l][b], 0, 0, 0);
       }
-      if (MODE != 1 && chunk_end && j < NL) {
+      if (MODE != 1 && j < NL && (last_stage || (st1 & (kc_stages(j) - 1)) == 0)) {
         // level j has received its last product of this chunk: flush it exactly
         constexpr double kScale[4] = {1.0, 0x1.0p-8, 0x1.0p-16, 0x1.0p-24};
 #pragma unroll

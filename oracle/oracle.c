@@ -404,21 +404,24 @@ ORC_API void orc_sgemm_dot2(char ta, char tb, int64_t m, int64_t n, int64_t k, d
  *   x = ldexp(a, -E_i) in (-1, 1);  r = x*2^7;  d_1 = rint(r); r -= d_1;
  *   for t >= 2: r *= 2^8; d_t = rint(r); r -= d_t.      (|d_t| <= 128)
  *   so  a ~= 2^E_i * sum_t d_t * 2^(-7-8(t-1)).
- *   Level sums over one k-chunk [c0, c0+kc):
+ *   Level sums over k-chunks of their own length kc_l:
  *     S_l = sum_{t+u=l, t,u<=s} sum_{p in chunk} dA_t[i,p]*dB_u[p,j]   (exact integers),
- *   for l = 2..s+1 (triangular truncation).  kc = largest power of two with
- *   kc * s <= 1024, so that every level sum of a chunk stays below 2^24 and is
+ *   for l = 2..s+1 (triangular truncation).  Level l has l-1 pairs and
+ *   kc_l = largest power of two with kc_l * (l-1) <= 1024 (1024, 512, 256,
+ *   256, 128, ...), so that every level sum of a chunk stays within 2^24 and is
  *   exact in FP32 whatever the summation order.
- *   After each chunk, in increasing l:  g = (l-2)/4, j = (l-2)%4,
- *     G_g = fma((double)S_l, 2^(-8j), G_g)            (groups of four levels).
+ *   At every multiple of the shortest chunk and at k, in increasing l, each
+ *   level whose chunk ends there is flushed:  g = (l-2)/4, j = (l-2)%4,
+ *     G_g = fma((double)S_l, 2^(-8j), G_g), S_l = 0     (groups of four levels).
  *   v = compensated sum over g from the smallest: w_g = ldexp(G_g, -32g);
  *       (hi, e) = TwoSum(hi, w_g), lo += e;  v = hi + lo;
  *   acc = ldexp(v, E_i + F_j - 14);  c = alpha*acc;  c = fma(beta, c_old, c) if beta != 0.
  * Rows/columns containing a non-finite value make the routine return 1.
  */
-ORC_API int orc_ozaki_chunk(int s) {
+/* kc_l of the 0-based level L = l - 2 (L + 1 pairs). */
+ORC_API int orc_ozaki_level_chunk(int L) {
   int kc = 1024;
-  while (kc * s > 1024) kc >>= 1;
+  while (kc * (L + 1) > 1024) kc >>= 1;
   return kc;
 }
 static int row_exponent(const double* v, int64_t n, int64_t stride, int* e) {
@@ -471,13 +474,17 @@ ORC_API int orc_ozaki_split_b(char tb, int64_t k, int64_t n, const double* B, in
   return 0;
 }
 
-static double ozaki_entry(int64_t m, int64_t n, int64_t k, int s, int64_t kc, const int* eA, const int16_t* dA,
+/* kcl[L]: chunk length of 0-based level L, each a multiple of the shortest. */
+static double ozaki_entry(int64_t m, int64_t n, int64_t k, int s, const int64_t* kcl, const int* eA, const int16_t* dA,
                           const int* eB, const int16_t* dB, int64_t i, int64_t j) {
   const int ngroups = (s + 3) / 4; /* levels 2..s+1 -> groups of 4 */
   double G[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int64_t c0 = 0; c0 < k; c0 += kc) {
-    const int64_t c1 = c0 + kc < k ? c0 + kc : k;
-    int64_t S[20] = {0};
+  int64_t S[20] = {0};
+  int64_t kmin = kcl[0];
+  for (int L = 1; L < s; ++L)
+    if (kcl[L] < kmin) kmin = kcl[L];
+  for (int64_t c0 = 0; c0 < k; c0 += kmin) {
+    const int64_t c1 = c0 + kmin < k ? c0 + kmin : k;
     for (int t = 1; t <= s; ++t)
       for (int u = 1; u + t <= s + 1; ++u) {
         const int16_t* a = dA + (int64_t)(t - 1) * m * k + i * k;
@@ -487,8 +494,10 @@ static double ozaki_entry(int64_t m, int64_t n, int64_t k, int s, int64_t kc, co
         S[t + u] += acc;
       }
     for (int l = 2; l <= s + 1; ++l) {
+      if (c1 % kcl[l - 2] != 0 && c1 != k) continue;
       const int g = (l - 2) / 4, jj = (l - 2) % 4;
       G[g] = fma((double)S[l], ldexp(1.0, -8 * jj), G[g]);
+      S[l] = 0;
     }
   }
   double hi = 0.0, lo = 0.0;
@@ -504,6 +513,8 @@ ORC_API int orc_dgemm_ozaki(char ta, char tb, int64_t m, int64_t n, int64_t k, d
                             const double* A, int64_t lda, const double* B, int64_t ldb,
                             double beta, double* C, int64_t ldc, int s, int nthreads) {
   if (s < 1 || s > 12) return -16;
+  int64_t kcl[12];
+  for (int L = 0; L < s; ++L) kcl[L] = orc_ozaki_level_chunk(L);
   (void)nthreads;
   if (alpha == 0.0 || k == 0) {
     for (int64_t j = 0; j < n; ++j)
@@ -523,7 +534,7 @@ ORC_API int orc_dgemm_ozaki(char ta, char tb, int64_t m, int64_t n, int64_t k, d
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
   for (int64_t j = 0; j < n; ++j)
     for (int64_t i = 0; i < m; ++i) {
-      double acc = ozaki_entry(m, n, k, s, orc_ozaki_chunk(s), eA, dA, eB, dB, i, j);
+      double acc = ozaki_entry(m, n, k, s, kcl, eA, dA, eB, dB, i, j);
       double c = alpha * acc;
       if (beta != 0.0) c = fma(beta, C[i + j * ldc], c);
       C[i + j * ldc] = c;
@@ -607,6 +618,8 @@ ORC_API int orc_dgemm_ozaki_i8(char ta, char tb, int64_t m, int64_t n, int64_t k
                                const double* A, int64_t lda, const double* B, int64_t ldb,
                                double beta, double* C, int64_t ldc, int s, int nthreads) {
   if (s < 1 || s > 8) return -16;
+  int64_t kcl[8];
+  for (int L = 0; L < s; ++L) kcl[L] = ORC_I8_CHUNK;
   if (alpha == 0.0 || k == 0) {
     for (int64_t j = 0; j < n; ++j)
       for (int64_t i = 0; i < m; ++i) C[i + j * ldc] = beta == 0.0 ? 0.0 : beta * C[i + j * ldc];
@@ -625,7 +638,7 @@ ORC_API int orc_dgemm_ozaki_i8(char ta, char tb, int64_t m, int64_t n, int64_t k
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
   for (int64_t j = 0; j < n; ++j)
     for (int64_t i = 0; i < m; ++i) {
-      double acc = ozaki_entry(m, n, k, s, ORC_I8_CHUNK, eA, dA, eB, dB, i, j);
+      double acc = ozaki_entry(m, n, k, s, kcl, eA, dA, eB, dB, i, j);
       double c = alpha * acc;
       if (beta != 0.0) c = fma(beta, C[i + j * ldc], c);
       C[i + j * ldc] = c;

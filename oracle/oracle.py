@@ -63,8 +63,8 @@ def lib():
     L.orc_ozaki_split_a.argtypes = [C.c_char, i64, i64, p, i64, C.c_int, p, p]
     L.orc_ozaki_split_b.argtypes = [C.c_char, i64, i64, p, i64, C.c_int, p, p]
     L.orc_dgemm_ozaki.argtypes = [C.c_char, C.c_char, i64, i64, i64, dbl, p, i64, p, i64, dbl, p, i64, C.c_int, C.c_int]
-    L.orc_ozaki_chunk.argtypes = [C.c_int]
-    L.orc_ozaki_chunk.restype = C.c_int
+    L.orc_ozaki_level_chunk.argtypes = [C.c_int]
+    L.orc_ozaki_level_chunk.restype = C.c_int
     L.orc_ozaki_i8_split_a.argtypes = [C.c_char, i64, i64, p, i64, C.c_int, p, p]
     L.orc_ozaki_i8_split_b.argtypes = [C.c_char, i64, i64, p, i64, C.c_int, p, p]
     L.orc_dgemm_ozaki_i8.argtypes = [C.c_char, C.c_char, i64, i64, i64, dbl, p, i64, p, i64, dbl, p, i64, C.c_int,
@@ -246,9 +246,10 @@ def dgemm_ozaki(ta, tb, alpha, A, B, beta, C, s, nthreads=None) -> np.ndarray:
     return C
 
 
-def ozaki_chunk(s: int) -> int:
-    """k-chunk length over which the emulated path's FP32 level sums stay exact."""
-    return int(lib().orc_ozaki_chunk(s))
+def ozaki_level_chunk(L: int) -> int:
+    """k-chunk length over which the FP32 sum of 0-based level L (L+1 slice
+    pairs) stays exact in the split-BF16 path."""
+    return int(lib().orc_ozaki_level_chunk(L))
 
 
 def ozaki_i8_split_a(ta, A, s):

@@ -291,6 +291,19 @@ def test_ozaki_exact_when_representable():
         assert np.array_equal(got, A @ B)  # integer-scaled inputs: A@B is exact in float64
 
 
+def test_ozaki_level_chunks_are_the_longest_exact_ones():
+    # level L has L+1 slice pairs of products |d_t d_u| <= 2^14: its FP32 sum over
+    # kc k-values is exact while (L+1) * kc * 2^14 <= 2^24; kc is the largest power of two
+    for L in range(12):
+        kc = O.ozaki_level_chunk(L)
+        assert kc & (kc - 1) == 0
+        assert (L + 1) * kc * 2 ** 14 <= 2 ** 24 < (L + 1) * 2 * kc * 2 ** 14
+    # the worst case (every digit 128) over one chunk is exactly representable in float32
+    L = 3
+    total = (L + 1) * O.ozaki_level_chunk(L) * 128 * 128
+    assert float(np.float32(total)) == total == 2 ** 24
+
+
 def test_ozaki_error_shrinks_per_slice():
     A = G.matrix(42, 48, 256, kind="wide")
     B = G.matrix(43, 256, 40, kind="wide")
