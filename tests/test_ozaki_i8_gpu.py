@@ -91,6 +91,25 @@ def test_ozaki_i8_exact_for_representable_inputs(gpu):
     assert np.array_equal(C, A @ B)
 
 
+def test_ozaki_i8_degenerate_sizes_and_alpha_zero(gpu):
+    # k == 0 or alpha == 0: C = beta*C without reading A or B; m == 0 / n == 0: no-op
+    C0 = G.matrix(7, 20, 10)
+    C = C0.copy(order="F")
+    jb.gemm_("N", "N", 1.0, np.zeros((20, 0), order="F"), np.zeros((0, 10), order="F"), 0.5, C,
+             method="ozaki_int8", nslices=7)
+    assert np.array_equal(C, 0.5 * C0)
+    C = C0.copy(order="F")
+    A = np.full((20, 30), np.nan, order="F")
+    jb.gemm_("N", "N", 0.0, A, G.matrix(8, 30, 10), 2.0, C, method="ozaki_int8", nslices=7)
+    assert np.array_equal(C, 2.0 * C0)
+    jb.gemm_("N", "N", 1.0, np.zeros((0, 5), order="F"), np.zeros((5, 3), order="F"), 0.0,
+             np.zeros((0, 3), order="F"), method="ozaki_int8", nslices=3)
+    with pytest.raises(jb.JBError) as ei:
+        jb.gemm_("N", "N", 1.0, G.matrix(1, 4, 4), G.matrix(2, 4, 4), 0.0, np.zeros((4, 4), order="F"),
+                 method="ozaki_int8", nslices=9)
+    assert ei.value.code == -14
+
+
 def test_ozaki_i8_nonfinite_is_an_error(gpu):
     A = G.matrix(1, 40, 30)
     B = G.matrix(2, 30, 20)
