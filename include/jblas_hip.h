@@ -107,6 +107,24 @@ int jb_jmul(int64_t M, int64_t N, int64_t P, double* D, const double* A, const d
 int jb_jmul_dev(int64_t M, int64_t N, int64_t P, double* D, const double* A, const double* X,
                 void* stream);
 
+/* ---- kernel! / initkernel! drop-in ---------------------------------------------- */
+/* kernel!(pD, pA, pX, Kernel{M,P,stride_AD,stride_X,N}) src/kernels.jl:212-241:
+ * D (M x P) += A (M x N) * X (N x P) with each element's chain starting from D:
+ * acc = D[i,j]; acc = fma(A[i,n], X[n,j], acc) for n = 1..N (the vload of D
+ * then the vmuladd k-loop, src/kernels.jl:225-233).  Column-major with leading
+ * dimensions ldd, lda, ldx (the reference shares one stride between A and D);
+ * N == 0 leaves D unchanged.  Host pointers, synchronous. */
+int jb_kernel(int64_t M, int64_t N, int64_t P, double* D, int64_t ldd, const double* A, int64_t lda,
+              const double* X, int64_t ldx);
+int jb_kernel_dev(int64_t M, int64_t N, int64_t P, double* D, int64_t ldd, const double* A,
+                  int64_t lda, const double* X, int64_t ldx, void* stream);
+/* initkernel!(pD, pA, pX, K) src/kernels.jl:242-275: D = A * X, the first
+ * k-step a plain product (evmul) then fma (vmuladd), as jmul!.  N >= 1. */
+int jb_initkernel(int64_t M, int64_t N, int64_t P, double* D, int64_t ldd, const double* A,
+                  int64_t lda, const double* X, int64_t ldx);
+int jb_initkernel_dev(int64_t M, int64_t N, int64_t P, double* D, int64_t ldd, const double* A,
+                      int64_t lda, const double* X, int64_t ldx, void* stream);
+
 /* ---- fastmul! drop-in ---------------------------------------------------------- */
 /* D (M x P) = A (M x N) * X (N x P), column-major, leading dimensions M, N, M, as
  * one small-GEMM launch: one wave per 64 rows x 16 columns of D, X read through

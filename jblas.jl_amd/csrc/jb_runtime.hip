@@ -441,6 +441,74 @@ int jb_jmul(int64_t M, int64_t N, int64_t P, double* D, const double* A, const d
   return jb_dgemm_recurrence('N', 'N', M, P, N, 1.0, A, M > 1 ? M : 1, X, N, 0.0, D, M > 1 ? M : 1);
 }
 
+// ---- kernel! / initkernel! (strided microkernels) --------------------------
+static int kernel_validate(bool init, int64_t M, int64_t N, int64_t P, const void* D, int64_t ldd, const void* A,
+                           int64_t lda, const void* X, int64_t ldx, GemmArgs* g) {
+  tl_error.clear();
+  if (M < 0) return fail(-1, "M < 0");
+  if (init ? N < 1 : N < 0) return fail(-2, init ? "N < 1 (initkernel! starts with a product)" : "N < 0");
+  if (P < 0) return fail(-3, "P < 0");
+  if (M > 0 && P > 0 && D == nullptr) return fail(-4, "D is NULL");
+  if (ldd < (M > 1 ? M : 1)) return fail(-5, "ldd < max(1, M)");
+  if (M > 0 && N > 0 && A == nullptr) return fail(-6, "A is NULL");
+  if (lda < (M > 1 ? M : 1)) return fail(-7, "lda < max(1, M)");
+  if (P > 0 && N > 0 && X == nullptr) return fail(-8, "X is NULL");
+  if (ldx < (N > 1 ? N : 1)) return fail(-9, "ldx < max(1, N)");
+  if (M > (int64_t)1 << 30 || P > (int64_t)1 << 30) return fail(-1, "M or P above 2^30 is not supported");
+  *g = GemmArgs{};
+  g->m = M;
+  g->n = P;
+  g->k = N;
+  g->alpha = 1.0;
+  g->beta = init ? 0.0 : 1.0;
+  g->A = A;
+  g->lda = lda;
+  g->B = X;
+  g->ldb = ldx;
+  g->C = const_cast<void*>(D);
+  g->ldc = ldd;
+  return JB_OK;
+}
+
+static int kernel_dev_locked(const GemmArgs& g, bool init, hipStream_t s) {
+  if (g.m == 0 || g.n == 0 || (!init && g.k == 0)) return JB_OK;
+  const hipError_t e = init ? launch_dgemm_recurrence(g, s) : launch_dgemm_kernel_acc(g, s);
+  return e == hipSuccess ? JB_OK : hip_fail(e, init ? "initkernel launch" : "kernel launch");
+}
+
+extern "C++" {
+static int kernel_entry(bool init, int64_t M, int64_t N, int64_t P, double* D, int64_t ldd, const double* A,
+                        int64_t lda, const double* X, int64_t ldx, void* stream, bool host) {
+  GemmArgs g;
+  int rc = kernel_validate(init, M, N, P, D, ldd, A, lda, X, ldx, &g);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(rt().mu);
+  if (M == 0 || P == 0 || (!init && N == 0)) return JB_OK;
+  if ((rc = ensure_init())) return rc;
+  if (!host) return kernel_dev_locked(g, init, (hipStream_t)stream);
+  return run_staged(g, sizeof(double), !init, [init](const GemmArgs& d, hipStream_t s) {
+    return kernel_dev_locked(d, init, s);
+  });
+}
+}  // extern "C++"
+
+int jb_kernel(int64_t M, int64_t N, int64_t P, double* D, int64_t ldd, const double* A, int64_t lda, const double* X,
+              int64_t ldx) {
+  return kernel_entry(false, M, N, P, D, ldd, A, lda, X, ldx, nullptr, true);
+}
+int jb_kernel_dev(int64_t M, int64_t N, int64_t P, double* D, int64_t ldd, const double* A, int64_t lda,
+                  const double* X, int64_t ldx, void* stream) {
+  return kernel_entry(false, M, N, P, D, ldd, A, lda, X, ldx, stream, false);
+}
+int jb_initkernel(int64_t M, int64_t N, int64_t P, double* D, int64_t ldd, const double* A, int64_t lda,
+                  const double* X, int64_t ldx) {
+  return kernel_entry(true, M, N, P, D, ldd, A, lda, X, ldx, nullptr, true);
+}
+int jb_initkernel_dev(int64_t M, int64_t N, int64_t P, double* D, int64_t ldd, const double* A, int64_t lda,
+                      const double* X, int64_t ldx, void* stream) {
+  return kernel_entry(true, M, N, P, D, ldd, A, lda, X, ldx, stream, false);
+}
+
 // ---- fastmul! (whole-matrix small GEMM) -----------------------------------
 extern "C++" {
 template <typename T>

@@ -248,7 +248,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
 constexpr int RB = 64;
 constexpr int RSTRIDE = RB + 2;
 
-template <bool TA, bool TB>
+// ACC: kernel! semantics (src/kernels.jl:225-233): each chain starts from the
+// element of C and every k-step is an fma; C = acc (alpha, beta unused).
+template <bool TA, bool TB, bool ACC>
 __global__ __launch_bounds__(NT, 2) void dgemm_recurrence_kernel(int64_t M, int64_t N, int64_t K, double alpha,
                                                                  const double* __restrict__ A, int64_t lda,
                                                                  const double* __restrict__ B, int64_t ldb,
@@ -269,7 +271,10 @@ __global__ __launch_bounds__(NT, 2) void dgemm_recurrence_kernel(int64_t M, int6
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    for (int j = 0; j < 4; ++j) {
+      const int64_t m = m0 + tx + 16 * i, n = n0 + ty + 16 * j;
+      acc[i][j] = (ACC && m < M && n < N) ? C[m + n * ldc] : 0.0;
+    }
 
   for (int64_t k0 = 0; k0 < K; k0 += BK) {
     la.load(A, lda, m0, k0, M, K, false);
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(NT, 2) void dgemm_recurrence_kernel(int64_t M, int6
     __syncthreads();
     const int kmax = (int)min<int64_t>(BK, K - k0);
     int kk = 0;
-    if (k0 == 0) {  // acc_1 = fl(a_1 * x_1)   (src/gemm.jl:86)
+    if (!ACC && k0 == 0) {  // acc_1 = fl(a_1 * x_1)   (src/gemm.jl:86)
       double a[4], b[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) a[i] = As[tx + 16 * i];
@@ -310,9 +315,13 @@ __global__ __launch_bounds__(NT, 2) void dgemm_recurrence_kernel(int64_t M, int6
     for (int j = 0; j < 4; ++j) {
       const int64_t m = m0 + tx + 16 * i, n = n0 + ty + 16 * j;
       if (m < M && n < N) {
-        double c = alpha * acc[i][j];
-        if (beta != 0.0) c = fma(beta, C[m + n * ldc], c);
-        C[m + n * ldc] = c;
+        if (ACC) {
+          C[m + n * ldc] = acc[i][j];
+        } else {
+          double c = alpha * acc[i][j];
+          if (beta != 0.0) c = fma(beta, C[m + n * ldc], c);
+          C[m + n * ldc] = c;
+        }
       }
     }
 }
@@ -379,24 +388,34 @@ static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
 hipError_t launch_dgemm_mfma(const GemmArgs& g, hipStream_t s) { return launch_gemm_mfma<double>(g, s); }
 hipError_t launch_sgemm_mfma(const GemmArgs& g, hipStream_t s) { return launch_gemm_mfma<float>(g, s); }
 
-hipError_t launch_dgemm_recurrence(const GemmArgs& g, hipStream_t s) {
+static hipError_t launch_recurrence(const GemmArgs& g, bool acc, hipStream_t s) {
   const int tiles_m = (int)((g.m + RB - 1) / RB), tiles_n = (int)((g.n + RB - 1) / RB);
   const int nblocks = tiles_m * tiles_n;
   auto A = static_cast<const double*>(g.A);
   auto B = static_cast<const double*>(g.B);
   auto C = static_cast<double*>(g.C);
-#define JB_LAUNCH(TA, TB)                                                                                 \
-  hipLaunchKernelGGL((dgemm_recurrence_kernel<TA, TB>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, g.alpha, \
+#define JB_LAUNCH(TA, TB, AC)                                                                                \
+  hipLaunchKernelGGL((dgemm_recurrence_kernel<TA, TB, AC>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, g.alpha, \
                      A, g.lda, B, g.ldb, g.beta, C, g.ldc, tiles_m, tiles_n)
   timing_begin(s, "dgemm_recurrence");
-  if (!g.ta && !g.tb) JB_LAUNCH(false, false);
-  else if (!g.ta && g.tb) JB_LAUNCH(false, true);
-  else if (g.ta && !g.tb) JB_LAUNCH(true, false);
-  else JB_LAUNCH(true, true);
+  if (acc) {
+    JB_LAUNCH(false, false, true);  // kernel! takes A and X as stored
+  } else if (!g.ta && !g.tb) {
+    JB_LAUNCH(false, false, false);
+  } else if (!g.ta && g.tb) {
+    JB_LAUNCH(false, true, false);
+  } else if (g.ta && !g.tb) {
+    JB_LAUNCH(true, false, false);
+  } else {
+    JB_LAUNCH(true, true, false);
+  }
 #undef JB_LAUNCH
   timing_end(s, "dgemm_recurrence");
   return hipGetLastError();
 }
+
+hipError_t launch_dgemm_recurrence(const GemmArgs& g, hipStream_t s) { return launch_recurrence(g, false, s); }
+hipError_t launch_dgemm_kernel_acc(const GemmArgs& g, hipStream_t s) { return launch_recurrence(g, true, s); }
 
 hipError_t launch_scale_f64(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipStream_t s) {
   if (m == 0 || n == 0) return hipSuccess;

@@ -189,15 +189,31 @@ def fastmul_(D, A, X, *, stream=None):
     return D
 
 
+def _kernel_call(name, D, A, X, stream):
+    d, a, x = _Operand(D, True, "D"), _Operand(A, False, "A"), _Operand(X, False, "X")
+    M, N = a.rows, a.cols
+    if x.rows != N or d.rows != M or d.cols != x.cols:
+        raise DimensionMismatch(f"D is {d.rows}x{d.cols}, A is {M}x{N}, X is {x.rows}x{x.cols}")
+    if not (d.dtype == a.dtype == x.dtype == np.float64):
+        raise TypeError("kernel!/initkernel! take Float64 matrices")
+    P = x.cols
+    if d.device:
+        _lib.call(name + "_dev", M, N, P, d.ptr, d.ld, a.ptr, a.ld, x.ptr, x.ld, _stream_of(True, stream))
+    else:
+        _lib.call(name, M, N, P, d.ptr, d.ld, a.ptr, a.ld, x.ptr, x.ld)
+    return D
+
+
 def kernel_(D, A, X, *, stream=None):
-    """D += A*X on (sub)matrices with their own leading dimensions
-    (kernel!, src/kernels.jl:212-241): gemm! with alpha = beta = 1."""
-    return gemm_("N", "N", 1.0, A, X, 1.0, D, method="recurrence", stream=stream)
+    """D += A*X on (sub)matrices with their own leading dimensions (kernel!,
+    src/kernels.jl:212-241): each element's fma chain starts from D."""
+    return _kernel_call("jb_kernel", D, A, X, stream)
 
 
 def initkernel_(D, A, X, *, stream=None):
-    """D = A*X on (sub)matrices (initkernel!, src/kernels.jl:242-275)."""
-    return gemm_("N", "N", 1.0, A, X, 0.0, D, method="recurrence", stream=stream)
+    """D = A*X on (sub)matrices (initkernel!, src/kernels.jl:242-275): plain
+    first product, then fma, as jmul!."""
+    return _kernel_call("jb_initkernel", D, A, X, stream)
 
 
 # ---- seeded device generator (mrandn replacement, src/randmat.jl:11-14) ------

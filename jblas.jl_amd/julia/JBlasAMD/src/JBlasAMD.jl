@@ -12,8 +12,8 @@ jBLAS.jl's GEMM entry point.  Drop-ins:
     `:ozaki` (split-BF16, keyword `nslices`), `:ozaki_int8` (INT8 slices); Float32 `:native`, `:bf16x3`.
   * `fastmul!(D, A, X)` -- the same recurrence as one small-GEMM launch
     (src/kernels.jl:202-208), Float64 or Float32, dense matrices.
-  * `kernel!(D, A, X)` / `initkernel!(D, A, X)` -- D += A*X / D = A*X on
-    strided sub-blocks (src/kernels.jl:212-275).
+  * `kernel!(D, A, X)` / `initkernel!(D, A, X)` -- D += A*X (each chain
+    starting from D) / D = A*X on strided sub-blocks (src/kernels.jl:212-275).
   * `gemm(tA, tB, α, A, B)` -- non-mutating form with a ChainRulesCore
     `rrule` (package extension) backed by the library's pullback entry point.
   * `GroupedPlan` -- many GEMMs in one persistent, hipGraph-captured launch.
@@ -135,10 +135,32 @@ function gemm!(tA::AbstractChar, tB::AbstractChar, α::Real, A::StridedMatrix{Fl
     C
 end
 
-"D += A*X on strided sub-blocks (kernel!, src/kernels.jl:212-241)."
-kernel!(D, A, X) = gemm!('N', 'N', 1.0, A, X, 1.0, D; method = :recurrence)
-"D = A*X on strided sub-blocks (initkernel!, src/kernels.jl:242-275)."
-initkernel!(D, A, X) = gemm!('N', 'N', 1.0, A, X, 0.0, D; method = :recurrence)
+function _kernel_dims(D, A, X)
+    M, N = size(A)
+    N2, P = size(X)
+    (N == N2 && size(D) == (M, P)) || throw(DimensionMismatch("D $(size(D)), A $(size(A)), X $(size(X))"))
+    (colmajor(D) && colmajor(A) && colmajor(X)) || throw(ArgumentError("unit-stride columns required"))
+    M, N, P
+end
+
+"""D += A*X on strided sub-blocks (kernel!, src/kernels.jl:212-241): each
+element's fma chain starts from D."""
+function kernel!(D::StridedMatrix{Float64}, A::StridedMatrix{Float64}, X::StridedMatrix{Float64})
+    M, N, P = _kernel_dims(D, A, X)
+    check(:jb_kernel, ccall((:jb_kernel, libjblas), Cint,
+        (Int64, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64),
+        M, N, P, D, ld(D), A, ld(A), X, ld(X)))
+    D
+end
+
+"D = A*X on strided sub-blocks (initkernel!, src/kernels.jl:242-275), the jmul! recurrence."
+function initkernel!(D::StridedMatrix{Float64}, A::StridedMatrix{Float64}, X::StridedMatrix{Float64})
+    M, N, P = _kernel_dims(D, A, X)
+    check(:jb_initkernel, ccall((:jb_initkernel, libjblas), Cint,
+        (Int64, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64),
+        M, N, P, D, ld(D), A, ld(A), X, ld(X)))
+    D
+end
 
 "Non-mutating α*op(A)*op(B); differentiable through the ChainRulesCore extension."
 function gemm(tA::AbstractChar, tB::AbstractChar, α::Real, A::StridedMatrix{T}, B::StridedMatrix{T};

@@ -394,6 +394,19 @@ ORC_API void orc_sgemm_dot2(char ta, char tb, int64_t m, int64_t n, int64_t k, d
     }
 }
 
+/* kernel! (src/kernels.jl:212-241, kernel_quote :225-233): D += A*X with each
+ * element's chain starting from D -- acc = D[i,j]; acc = fma(A[i,n], X[n,j], acc)
+ * for n = 0..N-1 in increasing n; leading dimensions lda, ldx, ldd. */
+ORC_API void orc_kernel_acc_d(int64_t M, int64_t N, int64_t P, const double* A, int64_t lda, const double* X,
+                              int64_t ldx, double* D, int64_t ldd) {
+  for (int64_t j = 0; j < P; ++j)
+    for (int64_t i = 0; i < M; ++i) {
+      double acc = D[i + j * ldd];
+      for (int64_t n = 0; n < N; ++n) acc = fma(A[i + n * lda], X[n + j * ldx], acc);
+      D[i + j * ldd] = acc;
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* 5. Split-BF16 emulated DGEMM, exact restatement                           */
 /* ------------------------------------------------------------------------ */

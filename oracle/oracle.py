@@ -71,6 +71,8 @@ def lib():
                                      C.c_int]
     L.orc_ozaki_i8_chunk.argtypes = []
     L.orc_ozaki_i8_chunk.restype = C.c_int
+    L.orc_kernel_acc_d.argtypes = [i64, i64, i64, p, i64, p, i64, p, i64]
+    L.orc_kernel_acc_d.restype = None
     L.orc_pick_kernel_size.argtypes = [i64, i64, i64, p]
     L.orc_blocking_structure.argtypes = [i64, i64, i64, i64, p, i64, i64, p]
     for fn in ("orc_jmul_d", "orc_jmul_s", "orc_dgemm_ozaki", "orc_ozaki_split_a", "orc_ozaki_split_b",
@@ -138,6 +140,16 @@ def jmul_naive(A, X) -> np.ndarray:
     P = X.shape[1]
     D = np.zeros((M, P), order="F")
     lib().orc_jmul_naive_d(M, N, P, _ptr(A), _ld(A), _ptr(X), _ld(X), _ptr(D), _ld(D))
+    return D
+
+
+def kernel_acc(D, A, X) -> np.ndarray:
+    """kernel! restatement: returns D + A*X with each chain starting from D."""
+    A, X = _f(A), _f(X)
+    D = np.array(D, dtype=np.float64, order="F", copy=True)
+    M, N = A.shape
+    P = X.shape[1]
+    lib().orc_kernel_acc_d(M, N, P, _ptr(A), _ld(A), _ptr(X), _ld(X), _ptr(D), _ld(D))
     return D
 
 

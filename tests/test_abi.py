@@ -83,6 +83,12 @@ def test_other_entry_points_validate():
     assert L.jb_sgemm(b"N", b"Z", 4, 4, 4, 1.0, 1, 4, 1, 4, 0.0, 1, 4) == -2
     assert L.jb_jmul(4, 0, 4, 1, 1, 1) == -2
     assert L.jb_jmul(-1, 2, 4, 1, 1, 1) == -1
+    assert L.jb_kernel(4, 3, 4, 1, 3, 1, 4, 1, 3) == -5      # ldd < M
+    assert L.jb_kernel(4, 3, 4, 1, 4, 1, 2, 1, 3) == -7      # lda < M
+    assert L.jb_kernel(4, 3, 4, 1, 4, 1, 4, 1, 2) == -9      # ldx < N
+    assert L.jb_kernel(4, 0, 4, 1, 4, None, 4, None, 1) == 0  # N == 0: D unchanged, nothing read
+    assert L.jb_initkernel(4, 0, 4, 1, 4, 1, 4, 1, 1) == -2  # initkernel! needs a product
+    assert L.jb_kernel(4, 3, 4, None, 4, 1, 4, 1, 3) == -4
     assert L.jb_fastmul(4, 0, 4, 1, 1, 1) == -2
     assert L.jb_fastmul_f32(4, 3, -4, 1, 1, 1) == -3
     assert L.jb_fastmul(4, 3, 4, None, 1, 1) == -4

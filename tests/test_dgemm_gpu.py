@@ -195,11 +195,20 @@ def test_strided_subviews(gpu):
     B = big[20:100, 100:170]    # 80 x 70
     Cbig = G.matrix(8, 301, 90)
     C = Cbig[1:101, 3:73]       # 100 x 70, ldc 301
-    ref = O.gemm_recurrence("N", "N", 1.0, A, B, 1.0, C)
+    ref_k = O.kernel_acc(C, A, B)  # kernel!: chains start from D
     Cw = np.asfortranarray(Cbig.copy())
     jb.kernel_(Cw[1:101, 3:73], A, B)  # D += A*X
-    assert np.array_equal(Cw[1:101, 3:73], ref)
+    assert np.array_equal(Cw[1:101, 3:73], ref_k)
     assert np.array_equal(Cw[0, :], Cbig[0, :]) and np.array_equal(Cw[:, 80:], Cbig[:, 80:])
+    Cd0 = to_dev(gpu, Cbig)
+    jb.kernel_(Cd0[1:101, 3:73], to_dev(gpu, big)[10:110, 5:85], to_dev(gpu, big)[20:100, 100:170])
+    gpu.cuda.synchronize()
+    assert np.array_equal(to_host(Cd0)[1:101, 3:73], ref_k)
+    Ci = np.asfortranarray(np.full((301, 90), np.nan))
+    jb.initkernel_(Ci[1:101, 3:73], A, B)  # D = A*X, the jmul! recurrence
+    assert np.array_equal(Ci[1:101, 3:73], O.jmul(A, B))
+    assert np.all(np.isnan(Ci[0, :])) and np.all(np.isnan(Ci[:, 80:]))
+    ref = O.gemm_recurrence("N", "N", 1.0, A, B, 1.0, C)
     bigd = to_dev(gpu, big)
     Cd = to_dev(gpu, Cbig)
     jb.gemm_("N", "N", 1.0, bigd[10:110, 5:85], bigd[20:100, 100:170], 1.0, Cd[1:101, 3:73])

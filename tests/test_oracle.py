@@ -141,6 +141,34 @@ def test_jmul_float32_bruteforce():
     assert D1[0, 0] == np.float32(2.0 ** -24)
 
 
+def test_kernel_acc_chain_starts_from_d():
+    # kernel! (src/kernels.jl:225-233): acc = D; acc = fma(a_n, x_n, acc) for every n.
+    # Brute force with exact rationals and one binary64 rounding per step:
+    rng = np.random.default_rng(30)
+    A = rng.uniform(-1, 1, (5, 7))
+    X = rng.uniform(-1, 1, (7, 4))
+    D0 = rng.uniform(-1, 1, (5, 4))
+    D = O.kernel_acc(D0, A, X)
+    for i in range(5):
+        for j in range(4):
+            acc = float(D0[i, j])
+            for n in range(7):
+                acc = float(Fraction(float(A[i, n])) * Fraction(float(X[n, j])) + Fraction(acc))
+            assert D[i, j] == acc
+    # unlike gemm!(beta = 1), whose chain starts with a plain product and adds D last
+    A1, X1, D1 = np.array([[2.0 ** -53, 2.0 ** -53]]), np.array([[1.0], [1.0]]), np.ones((1, 1))
+    assert O.kernel_acc(D1, A1, X1)[0, 0] == 1.0
+    assert O.gemm_recurrence("N", "N", 1.0, A1, X1, 1.0, D1)[0, 0] == 1.0 + 2.0 ** -52
+    # strided operands: only the addressed block changes; N = 0 leaves D unchanged
+    big = G.matrix(31, 40, 40)
+    out = O.kernel_acc(big, big[:, :0], np.zeros((0, 40)))
+    assert np.array_equal(out, big)
+    Ii = rng.integers(-50, 50, (6, 9)).astype(float)
+    Xi = rng.integers(-50, 50, (9, 3)).astype(float)
+    Di = rng.integers(-50, 50, (6, 3)).astype(float)
+    assert np.array_equal(O.kernel_acc(Di, Ii, Xi), Di + Ii @ Xi)
+
+
 def test_gemm_recurrence_matches_jmul_and_ops():
     A = G.matrix(10, 33, 21)
     B = G.matrix(11, 21, 17)
