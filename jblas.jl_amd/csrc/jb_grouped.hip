@@ -545,6 +545,8 @@ int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& f
   // longest work first
   std::stable_sort(work.begin(), work.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
   P.nprobs = (int)probs.size();
+  if (work.size() >= ((size_t)1 << 31) || (size_t)counters >= ((size_t)1 << 31))
+    return fail(JB_ERR_UNSUPPORTED, "grouped plan: %zu work items is too many for one launch", work.size());
   P.nitems = (int)work.size();
   P.ncounters = counters + 1;
   const size_t ob_probs = 0;

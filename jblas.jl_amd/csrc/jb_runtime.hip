@@ -60,8 +60,12 @@ int validate_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, bool alpha_
   if (ldb < (brows > 1 ? brows : 1)) return fail(-10, "ldb < max(1, %lld)", (long long)brows);
   if (m > 0 && n > 0 && C == nullptr) return fail(-12, "C is NULL");
   if (ldc < (m > 1 ? m : 1)) return fail(-13, "ldc < max(1, %lld)", (long long)m);
-  // Tile-grid sizes are int32 inside the kernels.
+  // Tile-grid sizes are int32 inside the kernels, and a launch grid holds fewer
+  // than 2^32 work-items: the output may have at most 2^23 tiles of 64 x 64
+  // (2^35 elements, more than one GPU's memory holds in FP64).
   if (m > (int64_t)1 << 30 || n > (int64_t)1 << 30) return fail(-3, "m or n above 2^30 is not supported");
+  if (((m + 63) / 64) * ((n + 63) / 64) >= ((int64_t)1 << 23))
+    return fail(JB_ERR_UNSUPPORTED, "C of %lld x %lld is too large for one launch", (long long)m, (long long)n);
   out->ta = is_trans(ta);
   out->tb = is_trans(tb);
   out->m = m;
@@ -455,6 +459,8 @@ static int kernel_validate(bool init, int64_t M, int64_t N, int64_t P, const voi
   if (P > 0 && N > 0 && X == nullptr) return fail(-8, "X is NULL");
   if (ldx < (N > 1 ? N : 1)) return fail(-9, "ldx < max(1, N)");
   if (M > (int64_t)1 << 30 || P > (int64_t)1 << 30) return fail(-1, "M or P above 2^30 is not supported");
+  if (((M + 63) / 64) * ((P + 63) / 64) >= ((int64_t)1 << 23))
+    return fail(JB_ERR_UNSUPPORTED, "D of %lld x %lld is too large for one launch", (long long)M, (long long)P);
   *g = GemmArgs{};
   g->m = M;
   g->n = P;

@@ -89,6 +89,9 @@ def test_other_entry_points_validate():
     assert L.jb_kernel(4, 0, 4, 1, 4, None, 4, None, 1) == 0  # N == 0: D unchanged, nothing read
     assert L.jb_initkernel(4, 0, 4, 1, 4, 1, 4, 1, 1) == -2  # initkernel! needs a product
     assert L.jb_kernel(4, 3, 4, None, 4, 1, 4, 1, 3) == -4
+    # an output grid beyond one launch (2^23 tiles of 64 x 64) is refused before any HIP call
+    assert L.jb_dgemm(b"N", b"N", 1 << 20, 1 << 20, 4, 1.0, 1, 1 << 20, 1, 4, 0.0, 1, 1 << 20) == 6
+    assert L.jb_kernel(1 << 20, 4, 1 << 20, 1, 1 << 20, 1, 1 << 20, 1, 4) == 6
     assert L.jb_fastmul(4, 0, 4, 1, 1, 1) == -2
     assert L.jb_fastmul_f32(4, 3, -4, 1, 1, 1) == -3
     assert L.jb_fastmul(4, 3, 4, None, 1, 1) == -4
