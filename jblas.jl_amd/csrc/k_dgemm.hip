@@ -46,8 +46,19 @@ struct TileLoader {
   typedef typename Vec16<T>::type V;
   V v[PER];
 
+  // full: the whole R x BK tile lies inside the operand (block-uniform); one
+  // uniform branch picks the unguarded or the guarded loads.
   __device__ __forceinline__ void load(const T* __restrict__ P, int64_t ld, int64_t r0, int64_t k0, int64_t Rtot,
                                        int64_t K, bool full) {
+    if (full)
+      load_tile<true>(P, ld, r0, k0, Rtot, K);
+    else
+      load_tile<false>(P, ld, r0, k0, Rtot, K);
+  }
+
+  template <bool FULL>
+  __device__ __forceinline__ void load_tile(const T* __restrict__ P, int64_t ld, int64_t r0, int64_t k0,
+                                            int64_t Rtot, int64_t K) {
     const int t = threadIdx.x;
     if (MN_CONTIG) {
       constexpr int RV = R / VW;  // vectors per k column
@@ -56,9 +67,9 @@ struct TileLoader {
       for (int i = 0; i < PER; ++i) {
         const int kk = kb + (NT / RV) * i;
         const T* src = P + (r0 + r) + (k0 + kk) * ld;
-        if (full && VEC) {
+        if (FULL && VEC) {
           v[i] = *reinterpret_cast<const V*>(src);
-        } else if (full) {
+        } else if (FULL) {
 #pragma unroll
           for (int e = 0; e < VW; ++e) v[i][e] = src[e];
         } else {
@@ -74,9 +85,9 @@ struct TileLoader {
       for (int i = 0; i < PER; ++i) {
         const int r = rb + (NT / KV) * i;
         const T* src = P + (k0 + kk) + (r0 + r) * ld;
-        if (full && VEC) {
+        if (FULL && VEC) {
           v[i] = *reinterpret_cast<const V*>(src);
-        } else if (full) {
+        } else if (FULL) {
 #pragma unroll
           for (int e = 0; e < VW; ++e) v[i][e] = src[e];
         } else {
@@ -116,7 +127,9 @@ struct TileLoader {
 // (1) MFMA GEMM: 128x128 block tile, 4 waves of 64x64 (4x4 MFMA blocks of
 //     16x16), or 64x64 with waves of 32x32 when the 128-tile grid would
 //     leave the device's last round mostly idle; BK=16 per LDS stage, two
-//     LDS stages.  T = double uses
+//     LDS stages, global loads issued a full stage before their LDS store
+//     (the load latency, not the MFMA work, bounded the one-stage-ahead
+//     form).  T = double uses
 //     v_mfma_f64_16x16x4f64, T = float v_mfma_f32_16x16x4f32.
 // ---------------------------------------------------------------------------
 #ifndef DG_PAD_BYTES
@@ -185,28 +198,45 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
   lb.load(B, ldb, n0, 0, N, K, fullB && BK <= K);
   la.template store<STRIDE, SWA>(As[0]);
   lb.template store<STRIDE, SWB>(Bs[0]);
+  // Stage kt+1 travels in registers during stage kt: loaded as soon as stage
+  // kt's registers went to LDS (one stage earlier), stored to the other LDS
+  // buffer near the end of stage kt.
+  auto load_stage = [&](int kt) {
+    const int64_t k0 = (int64_t)kt * BK;
+    const bool fk = k0 + BK <= K;
+    la.load(A, lda, m0, k0, M, K, fullA && fk);
+    lb.load(B, ldb, n0, k0, N, K, fullB && fk);
+  };
+  if (nkt > 1) load_stage(1);
   __syncthreads();
+
+  // MFMA t of a stage covers k = 4t..4t+3; lane group g supplies k = 4t+g.
+  // Swizzled column (w0 + 16i + l16) ^ kswz(4t + g): with t < 4, kswz(4t + g) =
+  // 16 (t >> 1) + 8 (t & 1) + 4 (g >> 1), and w0 is a multiple of 32, so it is
+  // w0 + 16 (i ^ (t >> 1)) + (l16 ^ (8 (t & 1) + 4 (g >> 1))): a compile-time
+  // block offset plus one of two per-lane columns.
+  static_assert(BK == 16, "swizzle decomposition assumes four MFMA k-groups per stage");
+  const int sw0 = l16 ^ (4 * (g >> 1)), sw1 = l16 ^ (8 + 4 * (g >> 1));
+  auto frags = [&](const T* as, const T* bs, int t, T (&a)[WI], T (&b)[WI]) {
+    const int row = (4 * t + g) * STRIDE;
+    const int swc = (t & 1) ? sw1 : sw0;
+#pragma unroll
+    for (int i = 0; i < WI; ++i)
+      a[i] = as[row + (SWA ? wm0 + 16 * (i ^ (t >> 1)) + swc : wm0 + 16 * i + l16)];
+#pragma unroll
+    for (int j = 0; j < WI; ++j)
+      b[j] = bs[row + (SWB ? wn0 + 16 * (j ^ (t >> 1)) + swc : wn0 + 16 * j + l16)];
+  };
 
   for (int kt = 0; kt < nkt; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < nkt) {
-      const int64_t k0 = (int64_t)(kt + 1) * BK;
-      const bool fk = k0 + BK <= K;
-      la.load(A, lda, m0, k0, M, K, fullA && fk);
-      lb.load(B, ldb, n0, k0, N, K, fullB && fk);
-    }
     const T* as = As[cur];
     const T* bs = Bs[cur];
+    T a[2][WI], b[2][WI];
+    frags(as, bs, 0, a[0], b[0]);
 #pragma unroll
     for (int t = 0; t < BK / 4; ++t) {
-      // MFMA t covers k = 4t..4t+3; lane group g supplies k = 4t+g.
-      T a[WI], b[WI];
-#pragma unroll
-      for (int i = 0; i < WI; ++i)
-        a[i] = as[(4 * t + g) * STRIDE + ((wm0 + 16 * i + l16) ^ (SWA ? kswz(4 * t + g) : 0))];
-#pragma unroll
-      for (int j = 0; j < WI; ++j)
-        b[j] = bs[(4 * t + g) * STRIDE + ((wn0 + 16 * j + l16) ^ (SWB ? kswz(4 * t + g) : 0))];
+      if (t + 1 < BK / 4) frags(as, bs, t + 1, a[(t + 1) & 1], b[(t + 1) & 1]);
       // D'[p][q] = sum_k X[p][k] Y[k][q] with X = op(B)^T block (p -> n) and Y = op(A)^T
       // block (q -> m), so a 16-lane group holds 16 consecutive rows of one column of C
       // (coalesced column-major stores).  The f64 MFMA is a sequential fma chain over
@@ -215,15 +245,42 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
 #pragma unroll
       for (int i = 0; i < WI; ++i)
 #pragma unroll
-        for (int j = 0; j < WI; ++j) acc[i][j] = Mfma<T>::mma(b[j], a[i], acc[i][j]);
-    }
-    if (kt + 1 < nkt) {
-      la.template store<STRIDE, SWA>(As[cur ^ 1]);
-      lb.template store<STRIDE, SWB>(Bs[cur ^ 1]);
+        for (int j = 0; j < WI; ++j) acc[i][j] = Mfma<T>::mma(b[t & 1][j], a[t & 1][i], acc[i][j]);
+      // The next stage goes to the other LDS buffer (last read before the
+      // previous barrier) one MFMA group before the end, so its writes
+      // complete under the last group instead of in front of the barrier;
+      // the registers then take the stage after it.
+      if (t == BK / 4 - 2 && kt + 1 < nkt) {
+        la.template store<STRIDE, SWA>(As[cur ^ 1]);
+        lb.template store<STRIDE, SWB>(Bs[cur ^ 1]);
+        if (kt + 2 < nkt) load_stage(kt + 2);
+      }
     }
     __syncthreads();
   }
 
+  if (m0 + BT <= M && n0 + BT <= N) {
+    // Interior tile: for each 16-row strip, the beta*C loads are issued
+    // together, then the results stored.
+#pragma unroll
+    for (int i = 0; i < WI; ++i) {
+      T* Ci = C + (m0 + wm0 + 16 * i + l16);
+      T cv[WI][4];
+#pragma unroll
+      for (int j = 0; j < WI; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cv[j][r] = beta != T(0) ? Ci[(n0 + wn0 + 16 * j + Mfma<T>::prow(r, g)) * ldc] : T(0);
+#pragma unroll
+      for (int j = 0; j < WI; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          T c = alpha * acc[i][j][r];
+          if (beta != T(0)) c = fma(beta, cv[j][r], c);
+          Ci[(n0 + wn0 + 16 * j + Mfma<T>::prow(r, g)) * ldc] = c;
+        }
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < WI; ++i) {
     const int64_t m = m0 + wm0 + 16 * i + l16;
