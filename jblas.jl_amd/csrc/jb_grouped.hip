@@ -61,18 +61,30 @@ struct GItem {
 // ---------------------------------------------------------------------------
 // operand staging (generic strides, coalesced along the contiguous index)
 // ---------------------------------------------------------------------------
+// The thread index as an opaque value, read once per work item: offsets derived
+// from it are then formed inside the work loop of the persistent kernel rather
+// than hoisted out of it and held in registers for all eight tile variants at
+// once (which exhausts the register file).
+__device__ __forceinline__ int work_item_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 // RC: the R index is the contiguous one (stride 1), else the k index is.
 template <typename T, int R, int BK, bool RC>
 struct GLoader {
   static constexpr int PER = R * BK / GNT;
   T v[PER];
+  int tid;  // thread index (see work_item_tid)
+  __device__ explicit GLoader(int t) : tid(t) {}
   static __device__ __forceinline__ int row_of(int e) { return RC ? (e % R) : (e / BK); }
   static __device__ __forceinline__ int k_of(int e) { return RC ? (e / R) : (e % BK); }
   __device__ __forceinline__ void load(const T* __restrict__ P, int64_t ld, int64_t r0, int64_t k0, int64_t Rtot,
                                        int64_t Kend) {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      const int e = threadIdx.x + GNT * i;
+      const int e = tid + GNT * i;
       const int64_t gr = r0 + row_of(e), gk = k0 + k_of(e);
       v[i] = (gr < Rtot && gk < Kend) ? (RC ? P[gr + gk * ld] : P[gk + gr * ld]) : T(0);
     }
@@ -82,7 +94,7 @@ struct GLoader {
   __device__ __forceinline__ void store_kmajor(T* S) const {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      const int e = threadIdx.x + GNT * i;
+      const int e = tid + GNT * i;
       S[k_of(e) * STRIDE + row_of(e)] = v[i];
     }
   }
@@ -168,7 +180,8 @@ __device__ void native_tile(const GProb& P, int tm, int tn, int split, char* sme
   typedef typename GMfma<T>::acc_t acc_t;
   T* As = reinterpret_cast<T*>(smem);
   T* Bs = As + 2 * BK * SA;
-  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int tid = work_item_tid();
+  const int lane = tid % 64, wave = tid / 64;
   const int wm0 = (wave % 2) * (BM / 2), wn0 = (wave / 2) * (BN / 2);
   const int g = lane / 16, l16 = lane % 16;
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
@@ -176,8 +189,8 @@ __device__ void native_tile(const GProb& P, int tm, int tn, int split, char* sme
   const int64_t ke = min(P.k, kb + P.kchunk);
   const T* A = static_cast<const T*>(P.A);
   const T* B = static_cast<const T*>(P.B);
-  GLoader<T, BM, BK, !TA> la;  // op(A) rows contiguous when A is not transposed
-  GLoader<T, BN, BK, TB> lb;   // op(B) columns contiguous when B is transposed
+  GLoader<T, BM, BK, !TA> la(tid);  // op(A) rows contiguous when A is not transposed
+  GLoader<T, BN, BK, TB> lb(tid);   // op(B) columns contiguous when B is transposed
   acc_t acc[IM][JN];
 #pragma unroll
   for (int i = 0; i < IM; ++i)
@@ -189,13 +202,17 @@ __device__ void native_tile(const GProb& P, int tm, int tn, int split, char* sme
   lb.load(B, P.ldb, n0, kb, P.n, ke);
   la.template store_kmajor<SA>(As);
   lb.template store_kmajor<SB>(Bs);
+  // Stage kt+1 travels in registers during stage kt: loaded right after stage
+  // kt went to LDS, stored to the other buffer before stage kt's last MFMA
+  // group, so a load has a whole stage to arrive and the LDS writes finish
+  // under MFMAs.
+  if (nkt > 1) {
+    la.load(A, P.lda, m0, kb + BK, P.m, ke);
+    lb.load(B, P.ldb, n0, kb + BK, P.n, ke);
+  }
   __syncthreads();
   for (int kt = 0; kt < nkt; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < nkt) {
-      la.load(A, P.lda, m0, kb + (int64_t)(kt + 1) * BK, P.m, ke);
-      lb.load(B, P.ldb, n0, kb + (int64_t)(kt + 1) * BK, P.n, ke);
-    }
     const T* as = As + cur * BK * SA;
     const T* bs = Bs + cur * BK * SB;
 #pragma unroll
@@ -209,10 +226,14 @@ __device__ void native_tile(const GProb& P, int tm, int tn, int split, char* sme
       for (int i = 0; i < IM; ++i)
 #pragma unroll
         for (int j = 0; j < JN; ++j) acc[i][j] = GMfma<T>::mma(b[j], a[i], acc[i][j]);
-    }
-    if (kt + 1 < nkt) {
-      la.template store_kmajor<SA>(As + (cur ^ 1) * BK * SA);
-      lb.template store_kmajor<SB>(Bs + (cur ^ 1) * BK * SB);
+      if (t == BK / 4 - 2 && kt + 1 < nkt) {
+        la.template store_kmajor<SA>(As + (cur ^ 1) * BK * SA);
+        lb.template store_kmajor<SB>(Bs + (cur ^ 1) * BK * SB);
+        if (kt + 2 < nkt) {
+          la.load(A, P.lda, m0, kb + (int64_t)(kt + 2) * BK, P.m, ke);
+          lb.load(B, P.ldb, n0, kb + (int64_t)(kt + 2) * BK, P.n, ke);
+        }
+      }
     }
     __syncthreads();
   }
@@ -228,10 +249,10 @@ __device__ void native_tile(const GProb& P, int tm, int tn, int split, char* sme
 
 // Splits the staged FP32 values into hi/mid/lo BF16 planes ([plane][row][RS]).
 template <int R, int BK, int RS, bool RC>
-__device__ __forceinline__ void split3_store(const float (&v)[R * BK / GNT], uint16_t* S) {
+__device__ __forceinline__ void split3_store(const float (&v)[R * BK / GNT], uint16_t* S, int tid) {
 #pragma unroll
   for (int q = 0; q < R * BK / GNT; ++q) {
-    const int e = threadIdx.x + GNT * q;
+    const int e = tid + GNT * q;
     const int r = GLoader<float, R, BK, RC>::row_of(e);
     const int kk = GLoader<float, R, BK, RC>::k_of(e);
     float x = v[q];
@@ -254,7 +275,8 @@ __device__ void bf16x3_tile(const GProb& P, int tm, int tn, int split, char* sme
   constexpr int IM = BM / 32, JN = BN / 32;
   uint16_t* As = reinterpret_cast<uint16_t*>(smem);   // [3][BM][RS]
   uint16_t* Bs = As + 3 * BM * RS;                    // [3][BN][RS]
-  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int tid = work_item_tid();
+  const int lane = tid % 64, wave = tid / 64;
   const int wm0 = (wave % 2) * (BM / 2), wn0 = (wave / 2) * (BN / 2);
   const int g = lane / 16, l16 = lane % 16;
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
@@ -262,8 +284,8 @@ __device__ void bf16x3_tile(const GProb& P, int tm, int tn, int split, char* sme
   const int64_t ke = min(P.k, kb + P.kchunk);
   const float* A = static_cast<const float*>(P.A);
   const float* B = static_cast<const float*>(P.B);
-  GLoader<float, BM, BK, !TA> la;
-  GLoader<float, BN, BK, TB> lb;
+  GLoader<float, BM, BK, !TA> la(tid);
+  GLoader<float, BN, BK, TB> lb(tid);
   f4 acc[2][IM][JN];
 #pragma unroll
   for (int l = 0; l < 2; ++l)
@@ -272,14 +294,22 @@ __device__ void bf16x3_tile(const GProb& P, int tm, int tn, int split, char* sme
 #pragma unroll
       for (int j = 0; j < JN; ++j) acc[l][i][j] = f4{0, 0, 0, 0};
   const int nkt = (int)((ke - kb + BK - 1) / BK);
+  // The FP32 stage kt+1 is loaded into registers as soon as stage kt has been
+  // split into LDS, so its latency overlaps stage kt's products.
+  if (nkt > 0) {
+    la.load(A, P.lda, m0, kb, P.m, ke);
+    lb.load(B, P.ldb, n0, kb, P.n, ke);
+  }
   for (int kt = 0; kt < nkt; ++kt) {
-    const int64_t k0 = kb + (int64_t)kt * BK;
-    la.load(A, P.lda, m0, k0, P.m, ke);
-    lb.load(B, P.ldb, n0, k0, P.n, ke);
     __syncthreads();  // previous stage consumed
-    split3_store<BM, BK, RS, !TA>(la.v, As);
-    split3_store<BN, BK, RS, TB>(lb.v, Bs);
+    split3_store<BM, BK, RS, !TA>(la.v, As, tid);
+    split3_store<BN, BK, RS, TB>(lb.v, Bs, tid);
     __syncthreads();
+    if (kt + 1 < nkt) {
+      const int64_t k1 = kb + (int64_t)(kt + 1) * BK;
+      la.load(A, P.lda, m0, k1, P.m, ke);
+      lb.load(B, P.ldb, n0, k1, P.n, ke);
+    }
     bf8 a[3][IM], b[3][JN];
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
