@@ -502,6 +502,23 @@ int validate_descs(int method, int64_t count, const jb_gemm_desc* d) {
 
 size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Resident workgroups per CU of the persistent kernel of `kind`, from the
+// device's register and LDS budget for that kernel (queried once per kind on
+// the runtime's device; the persistent grid is this times the CU count).
+hipError_t grouped_occupancy(int kind, int* occ) {
+  static int cached[3] = {0, 0, 0};
+  if (cached[kind] == 0) {
+    int n = 0;
+    hipError_t e = kind == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<0>, GNT, 0)
+                   : kind == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<1>, GNT, 0)
+                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<2>, GNT, 0);
+    if (e != hipSuccess) return e;
+    cached[kind] = std::min(std::max(n, 1), 8);
+  }
+  *occ = cached[kind];
+  return hipSuccess;
+}
+
 // Builds the device-side problem/work lists of a plan (no launch).
 int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& forced, GroupedPlanImpl& P) {
   P.method = method;
@@ -610,7 +627,8 @@ int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& f
     return hip_fail(e, "grouped plan: upload problems");
   if (!items.empty() && (e = hipMemcpy(P.items, items.data(), items.size() * sizeof(GItem), hipMemcpyHostToDevice)) != hipSuccess)
     return hip_fail(e, "grouped plan: upload work list");
-  const int occ = 2;
+  int occ = 0;
+  if ((e = grouped_occupancy(P.kind, &occ)) != hipSuccess) return hip_fail(e, "grouped plan: occupancy query");
   P.grid = std::max(1, std::min(P.nitems, rt().info.compute_units * occ));
   return JB_OK;
 }
