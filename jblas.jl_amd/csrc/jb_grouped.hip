@@ -100,12 +100,7 @@ struct GLoader {
   }
 };
 
-__device__ __forceinline__ uint16_t bf16_rn(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf16_f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
 
 template <typename T> struct GMfma;
 template <> struct GMfma<double> {
@@ -247,23 +242,53 @@ __device__ void native_tile(const GProb& P, int tm, int tn, int split, char* sme
   splitk_combine<T, BM, BN>(P, tm, tn, counters, sflag);
 }
 
-// Splits the staged FP32 values into hi/mid/lo BF16 planes ([plane][row][RS]).
-template <int R, int BK, int RS, bool RC>
-__device__ __forceinline__ void split3_store(const float (&v)[R * BK / GNT], uint16_t* S, int tid) {
+// FP32 operand staging of the 3xBF16 tile in runs of four consecutive k of one
+// row (R x BK tile, NR runs per thread), split into hi/mid/lo BF16 planes
+// ([plane][row][RS]) with one 8-byte LDS store per plane and run.
+//   RC (rows contiguous): run q of thread t is row t % R, k-run t / R + (GNT / R) q
+//   otherwise:            row t / (BK / 4) + (4 GNT / BK) q, k-run t % (BK / 4)
+// Lanes with consecutive t take consecutive addresses of the contiguous index.
+template <int R, int BK, bool RC>
+struct GRun4 {
+  static constexpr int NR = R * BK / (4 * GNT);
+  static_assert(NR >= 1 && R * BK == 4 * GNT * NR, "tile must split into whole runs per thread");
+  static_assert(RC ? (GNT % R == 0 && (GNT / R) * NR == BK / 4) : ((4 * GNT / BK) * NR == R), "run map");
+  float v[NR][4];
+  int tid;
+  __device__ explicit GRun4(int t) : tid(t) {}
+  __device__ __forceinline__ int row(int q) const { return RC ? tid % R : tid / (BK / 4) + (4 * GNT / BK) * q; }
+  __device__ __forceinline__ int krun(int q) const { return RC ? tid / R + (GNT / R) * q : tid % (BK / 4); }
+  __device__ __forceinline__ void load(const float* __restrict__ P, int64_t ld, int64_t r0, int64_t k0, int64_t Rtot,
+                                       int64_t Kend) {
 #pragma unroll
-  for (int q = 0; q < R * BK / GNT; ++q) {
-    const int e = tid + GNT * q;
-    const int r = GLoader<float, R, BK, RC>::row_of(e);
-    const int kk = GLoader<float, R, BK, RC>::k_of(e);
-    float x = v[q];
+    for (int q = 0; q < NR; ++q) {
+      const int64_t gr = r0 + row(q), gk = k0 + 4 * krun(q);
 #pragma unroll
-    for (int p = 0; p < 3; ++p) {
-      const uint16_t h = bf16_rn(x);
-      S[(p * R + r) * RS + kk] = h;
-      x -= bf16_f(h);
+      for (int e = 0; e < 4; ++e)
+        v[q][e] = (gr < Rtot && gk + e < Kend) ? (RC ? P[gr + (gk + e) * ld] : P[gk + e + gr * ld]) : 0.0f;
     }
   }
-}
+  // x = hi + mid + lo + rest: each plane is x rounded to BF16 (nearest even,
+  // v_cvt_pk_bf16_f32), subtracted exactly before the next plane.
+  template <int RS>
+  __device__ __forceinline__ void store3(uint16_t* S) const {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      float x[4] = {v[q][0], v[q][1], v[q][2], v[q][3]};
+      uint16_t* dst = S + row(q) * RS + 4 * krun(q);
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        const bf2 h0 = {(__bf16)x[0], (__bf16)x[1]}, h1 = {(__bf16)x[2], (__bf16)x[3]};
+        *reinterpret_cast<uint2*>(dst + p * R * RS) =
+            uint2{__builtin_bit_cast(uint32_t, h0), __builtin_bit_cast(uint32_t, h1)};
+        x[0] -= (float)h0[0];
+        x[1] -= (float)h0[1];
+        x[2] -= (float)h1[0];
+        x[3] -= (float)h1[1];
+      }
+    }
+  }
+};
 
 // 3xBF16 tile for FP32 operands: the staged FP32 tile is split into hi/mid/lo
 // BF16 planes in LDS ([plane][row][32 k]); v_mfma_f32_16x16x32_bf16 computes the
@@ -273,8 +298,8 @@ __device__ void bf16x3_tile(const GProb& P, int tm, int tn, int split, char* sme
   constexpr int BK = 32;
   constexpr int RS = BK + 8;  // bf16 per LDS row (pad 16 B)
   constexpr int IM = BM / 32, JN = BN / 32;
-  uint16_t* As = reinterpret_cast<uint16_t*>(smem);   // [3][BM][RS]
-  uint16_t* Bs = As + 3 * BM * RS;                    // [3][BN][RS]
+  constexpr int BUF = 3 * (BM + BN) * RS;  // one stage: A planes [3][BM][RS], then B planes [3][BN][RS]
+  uint16_t* const S0 = reinterpret_cast<uint16_t*>(smem);
   const int tid = work_item_tid();
   const int lane = tid % 64, wave = tid / 64;
   const int wm0 = (wave % 2) * (BM / 2), wn0 = (wave / 2) * (BN / 2);
@@ -284,8 +309,8 @@ __device__ void bf16x3_tile(const GProb& P, int tm, int tn, int split, char* sme
   const int64_t ke = min(P.k, kb + P.kchunk);
   const float* A = static_cast<const float*>(P.A);
   const float* B = static_cast<const float*>(P.B);
-  GLoader<float, BM, BK, !TA> la(tid);
-  GLoader<float, BN, BK, TB> lb(tid);
+  GRun4<BM, BK, !TA> la(tid);
+  GRun4<BN, BK, TB> lb(tid);
   f4 acc[2][IM][JN];
 #pragma unroll
   for (int l = 0; l < 2; ++l)
@@ -294,22 +319,24 @@ __device__ void bf16x3_tile(const GProb& P, int tm, int tn, int split, char* sme
 #pragma unroll
       for (int j = 0; j < JN; ++j) acc[l][i][j] = f4{0, 0, 0, 0};
   const int nkt = (int)((ke - kb + BK - 1) / BK);
-  // The FP32 stage kt+1 is loaded into registers as soon as stage kt has been
-  // split into LDS, so its latency overlaps stage kt's products.
+  // Two LDS stages.  Stage kt+1 is split into the other buffer after stage
+  // kt's products are issued, and stage kt+2 is then loaded into the freed
+  // registers, so a load has a whole stage to arrive; one barrier per stage.
   if (nkt > 0) {
+    __syncthreads();  // LDS may still be read by the previous work item
     la.load(A, P.lda, m0, kb, P.m, ke);
     lb.load(B, P.ldb, n0, kb, P.n, ke);
+    la.template store3<RS>(S0);
+    lb.template store3<RS>(S0 + 3 * BM * RS);
+    if (nkt > 1) {
+      la.load(A, P.lda, m0, kb + BK, P.m, ke);
+      lb.load(B, P.ldb, n0, kb + BK, P.n, ke);
+    }
+    __syncthreads();
   }
   for (int kt = 0; kt < nkt; ++kt) {
-    __syncthreads();  // previous stage consumed
-    split3_store<BM, BK, RS, !TA>(la.v, As, tid);
-    split3_store<BN, BK, RS, TB>(lb.v, Bs, tid);
-    __syncthreads();
-    if (kt + 1 < nkt) {
-      const int64_t k1 = kb + (int64_t)(kt + 1) * BK;
-      la.load(A, P.lda, m0, k1, P.m, ke);
-      lb.load(B, P.ldb, n0, k1, P.n, ke);
-    }
+    const uint16_t* As = S0 + (kt & 1) * BUF;
+    const uint16_t* Bs = As + 3 * BM * RS;
     bf8 a[3][IM], b[3][JN];
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
@@ -332,6 +359,16 @@ __device__ void bf16x3_tile(const GProb& P, int tm, int tn, int split, char* sme
         acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[1][j], a[1][i], acc[1][i][j], 0, 0, 0);
         acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[0][j], a[2][i], acc[1][i][j], 0, 0, 0);
       }
+    if (kt + 1 < nkt) {
+      uint16_t* nxt = S0 + ((kt + 1) & 1) * BUF;  // last read in stage kt-1, before the previous barrier
+      la.template store3<RS>(nxt);
+      lb.template store3<RS>(nxt + 3 * BM * RS);
+      if (kt + 2 < nkt) {
+        la.load(A, P.lda, m0, kb + (int64_t)(kt + 2) * BK, P.m, ke);
+        lb.load(B, P.ldb, n0, kb + (int64_t)(kt + 2) * BK, P.n, ke);
+      }
+    }
+    __syncthreads();
   }
 #pragma unroll
   for (int i = 0; i < IM; ++i)
@@ -374,7 +411,7 @@ __device__ __forceinline__ void dispatch_tile(const GProb& P, int combo, int tm,
 template <int KIND>
 __global__ __launch_bounds__(GNT, 2) void grouped_kernel(const GProb* __restrict__ probs, const GItem* __restrict__ items,
                                                          int nitems, int* __restrict__ next, int* __restrict__ counters) {
-  constexpr int SMEM = (KIND == 2) ? (3 * (64 + 64) * 40 * 2) : (2 * 16 * (64 + 8) * 2 * (KIND == 0 ? 8 : 4));
+  constexpr int SMEM = (KIND == 2) ? (2 * 3 * (64 + 64) * 40 * 2) : (2 * 16 * (64 + 8) * 2 * (KIND == 0 ? 8 : 4));
   __shared__ __attribute__((aligned(16))) char smem[SMEM];
   __shared__ int s_item, s_flag;
   for (;;) {
