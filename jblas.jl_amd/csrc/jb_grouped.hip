@@ -242,9 +242,15 @@ __device__ void native_tile(const GProb& P, int tm, int tn, int split, char* sme
   splitk_combine<T, BM, BN>(P, tm, tn, counters, sflag);
 }
 
+// LDS rows of the 3xBF16 tile hold BK = 32 BF16 (64 bytes, four 16-byte chunks,
+// no padding); chunk c of row r is stored at chunk c ^ ((r >> 2) & 3), so the 16
+// rows that one 16-byte fragment read covers (same c) hit 16 distinct 4-bank
+// groups of the 64-bank LDS.
+__device__ __forceinline__ int bf_chunk(int r, int c) { return c ^ ((r >> 2) & 3); }
+
 // FP32 operand staging of the 3xBF16 tile in runs of four consecutive k of one
 // row (R x BK tile, NR runs per thread), split into hi/mid/lo BF16 planes
-// ([plane][row][RS]) with one 8-byte LDS store per plane and run.
+// ([plane][row][BK], chunks swizzled) with one 8-byte LDS store per plane and run.
 //   RC (rows contiguous): run q of thread t is row t % R, k-run t / R + (GNT / R) q
 //   otherwise:            row t / (BK / 4) + (4 GNT / BK) q, k-run t % (BK / 4)
 // Lanes with consecutive t take consecutive addresses of the contiguous index.
@@ -270,16 +276,17 @@ struct GRun4 {
   }
   // x = hi + mid + lo + rest: each plane is x rounded to BF16 (nearest even,
   // v_cvt_pk_bf16_f32), subtracted exactly before the next plane.
-  template <int RS>
   __device__ __forceinline__ void store3(uint16_t* S) const {
+    static_assert(BK == 32, "four 16-byte chunks per LDS row");
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
       float x[4] = {v[q][0], v[q][1], v[q][2], v[q][3]};
-      uint16_t* dst = S + row(q) * RS + 4 * krun(q);
+      const int r = row(q), kr = krun(q);
+      uint16_t* dst = S + r * BK + 8 * bf_chunk(r, kr >> 1) + 4 * (kr & 1);
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
         const bf2 h0 = {(__bf16)x[0], (__bf16)x[1]}, h1 = {(__bf16)x[2], (__bf16)x[3]};
-        *reinterpret_cast<uint2*>(dst + p * R * RS) =
+        *reinterpret_cast<uint2*>(dst + p * R * BK) =
             uint2{__builtin_bit_cast(uint32_t, h0), __builtin_bit_cast(uint32_t, h1)};
         x[0] -= (float)h0[0];
         x[1] -= (float)h0[1];
@@ -296,7 +303,7 @@ struct GRun4 {
 template <int BM, int BN, bool TA, bool TB>
 __device__ void bf16x3_tile(const GProb& P, int tm, int tn, int split, char* smem, int* counters, int* sflag) {
   constexpr int BK = 32;
-  constexpr int RS = BK + 8;  // bf16 per LDS row (pad 16 B)
+  constexpr int RS = BK;  // bf16 per LDS row
   constexpr int IM = BM / 32, JN = BN / 32;
   constexpr int BUF = 3 * (BM + BN) * RS;  // one stage: A planes [3][BM][RS], then B planes [3][BN][RS]
   uint16_t* const S0 = reinterpret_cast<uint16_t*>(smem);
@@ -326,8 +333,8 @@ __device__ void bf16x3_tile(const GProb& P, int tm, int tn, int split, char* sme
     __syncthreads();  // LDS may still be read by the previous work item
     la.load(A, P.lda, m0, kb, P.m, ke);
     lb.load(B, P.ldb, n0, kb, P.n, ke);
-    la.template store3<RS>(S0);
-    lb.template store3<RS>(S0 + 3 * BM * RS);
+    la.store3(S0);
+    lb.store3(S0 + 3 * BM * RS);
     if (nkt > 1) {
       la.load(A, P.lda, m0, kb + BK, P.m, ke);
       lb.load(B, P.ldb, n0, kb + BK, P.n, ke);
@@ -342,10 +349,12 @@ __device__ void bf16x3_tile(const GProb& P, int tm, int tn, int split, char* sme
     for (int p = 0; p < 3; ++p) {
 #pragma unroll
       for (int i = 0; i < IM; ++i)
-        a[p][i] = __builtin_bit_cast(bf8, *reinterpret_cast<const u4*>(As + (p * BM + wm0 + 16 * i + l16) * RS + 8 * g));
+        a[p][i] = __builtin_bit_cast(bf8, *reinterpret_cast<const u4*>(As + (p * BM + wm0 + 16 * i + l16) * RS +
+                                                                          8 * bf_chunk(l16, g)));
 #pragma unroll
       for (int j = 0; j < JN; ++j)
-        b[p][j] = __builtin_bit_cast(bf8, *reinterpret_cast<const u4*>(Bs + (p * BN + wn0 + 16 * j + l16) * RS + 8 * g));
+        b[p][j] = __builtin_bit_cast(bf8, *reinterpret_cast<const u4*>(Bs + (p * BN + wn0 + 16 * j + l16) * RS +
+                                                                          8 * bf_chunk(l16, g)));
     }
 #pragma unroll
     for (int i = 0; i < IM; ++i)
@@ -361,8 +370,8 @@ __device__ void bf16x3_tile(const GProb& P, int tm, int tn, int split, char* sme
       }
     if (kt + 1 < nkt) {
       uint16_t* nxt = S0 + ((kt + 1) & 1) * BUF;  // last read in stage kt-1, before the previous barrier
-      la.template store3<RS>(nxt);
-      lb.template store3<RS>(nxt + 3 * BM * RS);
+      la.store3(nxt);
+      lb.store3(nxt + 3 * BM * RS);
       if (kt + 2 < nkt) {
         la.load(A, P.lda, m0, kb + (int64_t)(kt + 2) * BK, P.m, ke);
         lb.load(B, P.ldb, n0, kb + (int64_t)(kt + 2) * BK, P.n, ke);
@@ -411,7 +420,7 @@ __device__ __forceinline__ void dispatch_tile(const GProb& P, int combo, int tm,
 template <int KIND>
 __global__ __launch_bounds__(GNT, 2) void grouped_kernel(const GProb* __restrict__ probs, const GItem* __restrict__ items,
                                                          int nitems, int* __restrict__ next, int* __restrict__ counters) {
-  constexpr int SMEM = (KIND == 2) ? (2 * 3 * (64 + 64) * 40 * 2) : (2 * 16 * (64 + 8) * 2 * (KIND == 0 ? 8 : 4));
+  constexpr int SMEM = (KIND == 2) ? (2 * 3 * (64 + 64) * 32 * 2) : (2 * 16 * (64 + 8) * 2 * (KIND == 0 ? 8 : 4));
   __shared__ __attribute__((aligned(16))) char smem[SMEM];
   __shared__ int s_item, s_flag;
   for (;;) {
