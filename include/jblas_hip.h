@@ -245,7 +245,9 @@ int jb_autotune_grouped(int method, int64_t count, const jb_gemm_desc* descs, in
 /* Persist / restore the table as JSON (keyed by device, arch, method, bucket). */
 int jb_autotune_save(const char* path);
 int jb_autotune_load(const char* path);
-/* Query the choice for one problem shape: tile id and split-K. */
+/* Query the choice for one problem shape: tile id (0 = 64x64 MFMA tile, 1 = 32x32
+ * MFMA tile, 2 = the whole problem in one work item on the vector ALUs for
+ * m, n, k <= 64 -- fastmul!'s form, src/kernels.jl:202-208) and split-K. */
 int jb_autotune_lookup(int method, int64_t m, int64_t n, int64_t k, int* tile, int* splitk);
 /* Clear the table (static defaults apply again). */
 int jb_autotune_reset(void);

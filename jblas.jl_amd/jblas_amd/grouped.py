@@ -18,7 +18,9 @@ from . import _lib
 from .api import DimensionMismatch, _Operand, _stream_of, _trans
 
 METHODS = {"f64": _lib.JB_F64_NATIVE, "f32": _lib.JB_F32_NATIVE, "f32_bf16x3": _lib.JB_F32_BF16X3}
-TILES = {0: "64x64", 1: "32x32"}
+# work-item configurations: MFMA tiles, or the whole problem on the vector ALUs
+# (m, n, k <= 64, the fastmul! form)
+TILES = {0: "64x64", 1: "32x32", 2: "small"}
 
 
 def method_id(method) -> int:

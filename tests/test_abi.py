@@ -119,3 +119,18 @@ def test_no_device_is_a_clean_error():
         pytest.skip("a device is present")
     rc = _lib.lib().jb_init(0)
     assert rc == _lib.JB_ERR_HIP and "device" in _lib.last_error()
+
+
+def test_grouped_static_choice_rule():
+    """The planner's static choice (no tuning table; no GPU needed): the 64x64
+    MFMA tile from max(m, n) >= 96, else 32x32; a small output with a long k
+    splits k.  The small path (whole problem on the vector ALUs) is only taken
+    from a tuning-table entry, and only by problems with m, n, k <= 64."""
+    from jblas_amd import grouped as G
+    G.tune_reset()
+    for m, n, k in [(1, 1, 1), (64, 64, 64), (16, 40, 8), (64, 1, 64)]:
+        assert G.tune_lookup("f64", m, n, k) == ("32x32", 1), (m, n, k)
+        assert G.tune_lookup("f32_bf16x3", m, n, k) == ("32x32", 1), (m, n, k)
+    assert G.tune_lookup("f64", 96, 32, 64) == ("64x64", 1)
+    assert G.tune_lookup("f64", 64, 64, 4096) == ("32x32", 1)  # 4 tiles: no split
+    assert G.tune_lookup("f64", 16, 16, 4096) == ("32x32", 4)  # 1 tile: split while k / (2 s) >= 256 and tiles * s < 4

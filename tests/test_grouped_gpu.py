@@ -13,6 +13,7 @@ Bars:
     (h = 1e-6) on a 64^3 problem agree to 1e-7 relative (the loss is linear, so
     the difference quotient is exact up to rounding ~ u*|L|/h).
 """
+import math
 import os
 import tempfile
 
@@ -98,6 +99,85 @@ def test_grouped_forced_splitk_bitexact(gpu):
             ref = O.gemm_recurrence_chunked("N", "N", 1.0, to_host(A), to_host(B), 1.0, C0[i],
                                             jb.split_chunk(k, choices[i][1]))
             assert np.array_equal(to_host(C), ref), (m, n, k, choices[i])
+
+
+SMALL_SHAPES = [(1, 1, 1), (64, 64, 64), (33, 17, 5), (7, 64, 63), (64, 3, 1), (50, 40, 64)]
+TILE_SHAPES = [(200, 72, 96), (16, 16, 2048)]
+
+
+def small_group(torch, ta, tb, dtype, seed):
+    probs = []
+    for i, (m, n, k) in enumerate(SMALL_SHAPES + TILE_SHAPES):
+        A = dev_uniform(torch, m if ta == "N" else k, k if ta == "N" else m, seed + 3 * i, dtype)
+        B = dev_uniform(torch, k if tb == "N" else n, n if tb == "N" else k, seed + 1 + 3 * i, dtype)
+        C = dev_uniform(torch, m, n, seed + 2 + 3 * i, dtype)
+        probs.append((m, n, k, A, B, C))
+    return probs
+
+
+def load_small_table(tmpdir, method, shapes):
+    """A tuning table (the library's JSON format) naming the small path for the
+    size buckets of `shapes`: bucket (floor(log2(floor(sqrt(m n)))), floor(log2 k))."""
+    info = jb.device_info()
+    pre = f"{info['name']}|{info['arch']}|m{jb.grouped.method_id(method)}|"
+    keys = sorted({f"mn{max(math.isqrt(m * n), 1).bit_length() - 1}|k{max(k, 1).bit_length() - 1}"
+                   for m, n, k in shapes})
+    path = os.path.join(tmpdir, "small.json")
+    entries = ",\n".join(f'    {{"key": "{pre}{k}", "cfg": 2, "splitk": 1, "ms": 0.0}}' for k in keys)
+    with open(path, "w") as f:
+        f.write('{\n  "format": "jblas-tune-1",\n  "entries": [\n' + entries + "\n  ]\n}\n")
+    jb.tune_load(path)
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_grouped_small_path_f64_bitexact(gpu, ta, tb, tmp_path):
+    """m, n, k <= 64 with a table entry for the small path: the whole problem
+    in one work item on the vector ALUs (fastmul!'s form), launched together
+    with problems on MFMA tiles; FP64 results equal the reference recurrence
+    (plain first product, then fma in k order) bit for bit.  A problem of a
+    small-path bucket that does not fit (m = 65) takes the 32x32 tile."""
+    torch = gpu
+    jb.tune_reset()
+    load_small_table(str(tmp_path), "f64", SMALL_SHAPES)
+    assert jb.tune_lookup("f64", 65, 63, 64)[0] == "32x32"  # bucket of (50, 40, 64), m too large
+    probs = small_group(torch, ta, tb, torch.float64, 70 + ord(ta) + 2 * ord(tb))
+    kinds = [jb.tune_lookup("f64", m, n, k) for (m, n, k, *_r) in probs]
+    assert [t for t, _ in kinds[:len(SMALL_SHAPES)]] == ["small"] * len(SMALL_SHAPES)
+    assert all(t != "small" for t, _ in kinds[len(SMALL_SHAPES):]), kinds
+    C0 = [to_host(p[5]) for p in probs]
+    for alpha, beta in [(1.5, 0.5), (-1.0, 0.0), (0.0, 2.0)]:
+        for i, p in enumerate(probs):
+            p[5].copy_(to_dev(torch, C0[i]))
+        jb.gemm_grouped_("f64", [GemmDesc(ta, tb, alpha, A, B, beta, C) for (m, n, k, A, B, C) in probs])
+        torch.cuda.synchronize()
+        for i, (m, n, k, A, B, C) in enumerate(probs):
+            ref = O.gemm_recurrence_chunked(ta, tb, alpha, to_host(A), to_host(B), beta, C0[i],
+                                            jb.split_chunk(k, kinds[i][1]))
+            assert np.array_equal(to_host(C), ref), (ta, tb, m, n, k, kinds[i], alpha, beta)
+    jb.tune_reset()
+
+
+@pytest.mark.parametrize("method", ["f32", "f32_bf16x3"])
+def test_grouped_small_path_f32(gpu, method, tmp_path):
+    """FP32 methods take the same small path (an FP32 fma chain) from a table entry."""
+    torch = gpu
+    jb.tune_reset()
+    load_small_table(str(tmp_path), method, SMALL_SHAPES)
+    assert all(jb.tune_lookup(method, m, n, k)[0] == "small" for m, n, k in SMALL_SHAPES)
+    for ta, tb in [("N", "N"), ("T", "T")]:
+        probs = small_group(torch, ta, tb, torch.float32, 90 + ord(ta))
+        C0 = [to_host(p[5]) for p in probs]
+        jb.gemm_grouped_(method, [GemmDesc(ta, tb, 1.5, A, B, 0.5, C) for (m, n, k, A, B, C) in probs])
+        torch.cuda.synchronize()
+        for i, (m, n, k, A, B, C) in enumerate(probs):
+            Ah, Bh = to_host(A), to_host(B)
+            ref = O.sgemm_dot2(ta, tb, 1.5, Ah, Bh, 0.5, C0[i])
+            Ao = Ah if ta == "N" else Ah.T
+            Bo = Bh if tb == "N" else Bh.T
+            absab = 1.5 * (np.abs(Ao).astype(float) @ np.abs(Bo).astype(float)) + 0.5 * np.abs(C0[i])
+            err = np.max(np.abs(to_host(C).astype(float) - ref) / absab)
+            assert err <= 2 * k * U32, (method, ta, tb, m, n, k, err)
+    jb.tune_reset()
 
 
 def test_grouped_plan_graph_replays(gpu):
