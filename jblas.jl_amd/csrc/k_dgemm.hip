@@ -183,8 +183,13 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
   const int wm0 = (wave % 2) * (BT / 2), wn0 = (wave / 2) * (BT / 2);
   const int g = lane / 16, l16 = lane % 16;
 
-  TileLoader<T, BT, !TA, VEC> la;
-  TileLoader<T, BT, TB, VEC> lb;
+  // PD register sets carry the next PD stages: a stage's loads are issued PD
+  // stages before its LDS store.  The 64-tile runs where the grid has about one
+  // tile per CU and a stage is only 16 MFMAs per wave, far shorter than a
+  // global load, so it keeps 4 stages in flight; the 128-tile keeps 1.
+  constexpr int PD = BT == 64 ? 4 : 1;
+  TileLoader<T, BT, !TA, VEC> la[PD];
+  TileLoader<T, BT, TB, VEC> lb[PD];
   const bool fullA = m0 + BT <= M, fullB = n0 + BT <= N;
 
   acc_t acc[WI][WI];
@@ -194,20 +199,21 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
     for (int j = 0; j < WI; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   const int nkt = (int)((K + BK - 1) / BK);
-  la.load(A, lda, m0, 0, M, K, fullA && BK <= K);
-  lb.load(B, ldb, n0, 0, N, K, fullB && BK <= K);
-  la.template store<STRIDE, SWA>(As[0]);
-  lb.template store<STRIDE, SWB>(Bs[0]);
-  // Stage kt+1 travels in registers during stage kt: loaded as soon as stage
-  // kt's registers went to LDS (one stage earlier), stored to the other LDS
-  // buffer near the end of stage kt.
-  auto load_stage = [&](int kt) {
+  // Stage j travels in register set j % PD: loaded as soon as stage j - PD
+  // left that set for LDS, stored to LDS buffer j & 1 near the end of stage
+  // j - 1.
+  auto load_stage = [&](auto& xa, auto& xb, int kt) {
     const int64_t k0 = (int64_t)kt * BK;
     const bool fk = k0 + BK <= K;
-    la.load(A, lda, m0, k0, M, K, fullA && fk);
-    lb.load(B, ldb, n0, k0, N, K, fullB && fk);
+    xa.load(A, lda, m0, k0, M, K, fullA && fk);
+    xb.load(B, ldb, n0, k0, N, K, fullB && fk);
   };
-  if (nkt > 1) load_stage(1);
+  load_stage(la[0], lb[0], 0);
+  la[0].template store<STRIDE, SWA>(As[0]);
+  lb[0].template store<STRIDE, SWB>(Bs[0]);
+#pragma unroll
+  for (int j = 1; j <= PD; ++j)
+    if (j < nkt) load_stage(la[j % PD], lb[j % PD], j);
   __syncthreads();
 
   // MFMA t of a stage covers k = 4t..4t+3; lane group g supplies k = 4t+g.
@@ -228,7 +234,12 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
       b[j] = bs[row + (SWB ? wn0 + 16 * (j ^ (t >> 1)) + swc : wn0 + 16 * j + l16)];
   };
 
-  for (int kt = 0; kt < nkt; ++kt) {
+  for (int kt0 = 0; kt0 < nkt; kt0 += PD)
+#pragma unroll
+  for (int d = 0; d < PD; ++d) {
+    const int kt = kt0 + d;
+    if (kt >= nkt) break;
+    const int nset = (d + 1) % PD;  // register set of stage kt+1 (compile-time)
     const int cur = kt & 1;
     const T* as = As[cur];
     const T* bs = Bs[cur];
@@ -251,9 +262,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
       // complete under the last group instead of in front of the barrier;
       // the registers then take the stage after it.
       if (t == BK / 4 - 2 && kt + 1 < nkt) {
-        la.template store<STRIDE, SWA>(As[cur ^ 1]);
-        lb.template store<STRIDE, SWB>(Bs[cur ^ 1]);
-        if (kt + 2 < nkt) load_stage(kt + 2);
+        la[nset].template store<STRIDE, SWA>(As[cur ^ 1]);
+        lb[nset].template store<STRIDE, SWB>(Bs[cur ^ 1]);
+        if (kt + 1 + PD < nkt) load_stage(la[nset], lb[nset], kt + 1 + PD);
       }
     }
     __syncthreads();
