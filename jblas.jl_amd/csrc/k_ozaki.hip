@@ -86,29 +86,76 @@ __device__ __forceinline__ int exponent_of_bits(unsigned long long bits) {
 }
 
 // ---------------------------------------------------------------------------
-// 1. absmax: maxbits[r] = max_k bits(|X(r,k)|), X(r,k) = P[r*sr + k*sk].
-//    64 x 64 tiles, coalesced along whichever index is contiguous.
+// 1. absmax: maxbits[r] = max_k bits(|X(r,k)|), X(r,k) = P[r*sr + k*sk].  For
+//    non-negative doubles the bit pattern orders like the value, and Inf/NaN
+//    patterns exceed every finite one, so they reach the split's check.
+//    Register-only streaming reductions, one atomicMax per row and block:
+//    rows contiguous (sr == 1): lane = row, each thread AM_RKC columns;
+//    otherwise (k contiguous): a wave takes AM_WROWS rows, reading AM_KK
+//    consecutive k of a row per pass (64 lanes x 8 bytes), and reduces across
+//    its lanes.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void absmax_kernel(const double* __restrict__ P, int64_t sr, int64_t sk, int64_t R,
-                                                     int64_t K, unsigned long long* __restrict__ maxbits) {
-  __shared__ unsigned long long tile[64][65];
-  const int t = threadIdx.x;
-  const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
-  const bool rows_contig = (sr == 1);
-#pragma unroll 4
-  for (int j = 0; j < 16; ++j) {
-    const int r = rows_contig ? (t % 64) : (t / 64 + 4 * j);
-    const int k = rows_contig ? (t / 64 + 4 * j) : (t % 64);
-    unsigned long long b = 0ull;
-    if (r0 + r < R && k0 + k < K) b = (unsigned long long)__double_as_longlong(fabs(P[(r0 + r) * sr + (k0 + k) * sk]));
-    tile[r][k] = b;
+constexpr int AM_RKC = 256;   // columns per thread, rows contiguous
+constexpr int AM_WROWS = 16;  // rows per wave, k contiguous
+constexpr int AM_KK = 1024;   // k per block, k contiguous
+
+__device__ __forceinline__ unsigned long long abs_bits(double x) {
+  return (unsigned long long)__double_as_longlong(fabs(x));
+}
+
+__global__ __launch_bounds__(256) void absmax_rows_kernel(const double* __restrict__ P, int64_t sk, int64_t R,
+                                                          int64_t K, unsigned long long* __restrict__ maxbits) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= R) return;
+  const int64_t k0 = (int64_t)blockIdx.y * AM_RKC;
+  const int64_t nk = min<int64_t>(AM_RKC, K - k0);
+  const double* __restrict__ p = P + r + k0 * sk;
+  unsigned long long m = 0ull;
+  int64_t k = 0;
+  for (; k + 16 <= nk; k += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = p[(k + u) * sk];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) m = max(m, abs_bits(v[u]));
   }
-  __syncthreads();
-  if (t < 64 && r0 + t < R) {
+  for (; k < nk; ++k) m = max(m, abs_bits(p[k * sk]));
+  atomicMax(maxbits + r, m);
+}
+
+__global__ __launch_bounds__(256) void absmax_kcontig_kernel(const double* __restrict__ P, int64_t sr, int64_t sk,
+                                                             int64_t R, int64_t K,
+                                                             unsigned long long* __restrict__ maxbits) {
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int64_t rw = (int64_t)blockIdx.x * (4 * AM_WROWS) + (int64_t)wave * AM_WROWS;
+  const int64_t k0 = (int64_t)blockIdx.y * AM_KK;
+#pragma unroll 2
+  for (int i = 0; i < AM_WROWS; ++i) {
+    const int64_t r = rw + i;
+    if (r >= R) break;  // wave-uniform
+    const double* __restrict__ p = P + r * sr;
     unsigned long long m = 0ull;
-    for (int k = 0; k < 64; ++k) m = max(m, tile[t][k]);
-    atomicMax(maxbits + r0 + t, m);
+#pragma unroll
+    for (int j = 0; j < AM_KK / 64; ++j) {
+      const int64_t k = k0 + lane + 64 * j;
+      if (k < K) m = max(m, abs_bits(p[k * sk]));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned long long)__shfl_xor(m, off, 64));
+    if (lane == 0) atomicMax(maxbits + r, m);
   }
+}
+
+void launch_absmax(const double* P, int64_t sr, int64_t sk, int64_t R, int64_t K, unsigned long long* mb,
+                   hipStream_t st) {
+  if (R <= 0 || K <= 0) return;
+  if (sr == 1)
+    hipLaunchKernelGGL(absmax_rows_kernel, dim3((unsigned)((R + 255) / 256), (unsigned)((K + AM_RKC - 1) / AM_RKC)),
+                       dim3(256), 0, st, P, sk, R, K, mb);
+  else
+    hipLaunchKernelGGL(absmax_kcontig_kernel,
+                       dim3((unsigned)((R + 4 * AM_WROWS - 1) / (4 * AM_WROWS)), (unsigned)((K + AM_KK - 1) / AM_KK)),
+                       dim3(256), 0, st, P, sr, sk, R, K, mb);
 }
 
 __global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
@@ -588,8 +635,7 @@ void launch_split_f64(const double* P, int64_t sr, int64_t sk, int64_t R, int64_
                       unsigned long long* mb, int* ex, uint16_t* planes, int* flag, hipStream_t st) {
   hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)std::min<int64_t>((Rpad + 255) / 256, 1024)), dim3(256), 0, st,
                      mb, Rpad);
-  hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)((R + 63) / 64), (unsigned)((K + 63) / 64)), dim3(256), 0, st, P,
-                     sr, sk, R, K, mb);
+  launch_absmax(P, sr, sk, R, K, mb, st);
   hipLaunchKernelGGL(split_f64_kernel<S>, dim3((unsigned)(Rpad / 64), (unsigned)(Kpad / 64)), dim3(256), 0, st, P, sr,
                      sk, R, K, mb, ex, planes, Rpad * Kpad, Kpad, RB, flag);
 }
@@ -646,8 +692,7 @@ void launch_split_f64_i8(const double* P, int64_t sr, int64_t sk, int64_t R, int
                          int RB, unsigned long long* mb, int* ex, uint8_t* planes, int* flag, hipStream_t st) {
   hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)std::min<int64_t>((Rpad + 255) / 256, 1024)), dim3(256), 0, st,
                      mb, Rpad);
-  hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)((R + 63) / 64), (unsigned)((K + 63) / 64)), dim3(256), 0, st, P,
-                     sr, sk, R, K, mb);
+  launch_absmax(P, sr, sk, R, K, mb, st);
   hipLaunchKernelGGL(split_f64_i8_kernel<S>, dim3((unsigned)(Rpad / 64), (unsigned)(Kpad / 64)), dim3(256), 0, st, P,
                      sr, sk, R, K, mb, ex, planes, Rpad * Kpad, Kpad, RB, flag);
 }
