@@ -635,10 +635,15 @@ int validate_descs(int method, int64_t count, const jb_gemm_desc* d) {
 size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Resident workgroups per CU of the persistent kernel of `kind`, from the
-// device's register and LDS budget for that kernel (queried once per kind on
-// the runtime's device; the persistent grid is this times the CU count).
+// device's register and LDS budget for that kernel (queried once per kind and
+// device, under the runtime lock; the persistent grid is this times the CU count).
 hipError_t grouped_occupancy(int kind, int* occ) {
   static int cached[3] = {0, 0, 0};
+  static int cached_device = -1;
+  if (cached_device != rt().device) {
+    cached[0] = cached[1] = cached[2] = 0;
+    cached_device = rt().device;
+  }
   if (cached[kind] == 0) {
     int n = 0;
     hipError_t e = kind == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<0>, GNT, 0)

@@ -127,9 +127,9 @@ struct TileLoader {
 // (1) MFMA GEMM: 128x128 block tile, 4 waves of 64x64 (4x4 MFMA blocks of
 //     16x16), or 64x64 with waves of 32x32 when the 128-tile grid would
 //     leave the device's last round mostly idle; BK=16 per LDS stage, two
-//     LDS stages, global loads issued a full stage before their LDS store
-//     (the load latency, not the MFMA work, bounded the one-stage-ahead
-//     form).  T = double uses
+//     LDS stages; global loads are issued one full stage (128-tile) or four
+//     stages (64-tile) before their LDS store (the load latency, not the MFMA
+//     work, bounded the earlier form that loaded within the stage).  T = double uses
 //     v_mfma_f64_16x16x4f64, T = float v_mfma_f32_16x16x4f32.
 // ---------------------------------------------------------------------------
 #ifndef DG_PAD_BYTES
