@@ -396,52 +396,116 @@ __device__ void bf16x3_tile(const GProb& P, int tm, int tn, int split, char* sme
 }
 
 // Small problem (m, n, k <= 64) in one work item, the form of fastmul!
-// (src/kernels.jl:202-208, kernel_quote :42-201): no LDS staging and no MFMA
-// tile; lane = row of C (rows >= m recompute row m-1 and store nothing), wave w
-// holds columns 16w..16w+15 in registers (columns >= n re-read column n-1),
-// A(i, p) is one load per lane and B(p, j) a wave-uniform load.  Each element
-// is the chain acc = A(i,0) B(0,j), acc = fma(A(i,p), B(p,j), acc) for
-// p = 1..k-1 -- the plain first product of src/gemm.jl:86 and the k-steps of
-// :165 -- then the alpha/beta epilogue of the tiles (put_result).
+// (src/kernels.jl:202-208, kernel_quote :42-201): no MFMA tile; lane = row of C
+// (rows >= m recompute row m-1 and store nothing), wave w holds columns
+// 16w..16w+15 in registers.  The workgroup first stages all of op(B) (k x n)
+// in LDS, [p][j] with rows of SB_STRIDE elements, in two batches of loads;
+// each lane then reads its row of op(A) in register chunks of 4 k (the next
+// chunk in flight) and B(p, j0..j0+15) as LDS broadcasts, 8 columns at a time
+// (registers stay within the budget of the MFMA tiles sharing this kernel).  Each element is the chain
+// acc = A(i,0) B(0,j), acc = fma(A(i,p), B(p,j), acc) for p = 1..k-1 -- the
+// plain first product of src/gemm.jl:86 and the k-steps of :165 -- then the
+// alpha/beta epilogue of the tiles (put_result).
 template <typename T, bool TA, bool TB>
-__device__ void small_tile(const GProb& P) {
-  constexpr int SC = 16;  // columns per wave
+__device__ __forceinline__ void small_tile(const GProb& P, char* smem) {
+  constexpr int SC = 16;                          // columns per wave
+  constexpr int SB_STRIDE = 64 + 16 / sizeof(T);  // 16-byte aligned rows, 4-way at most on the staging writes
+  constexpr int AC = 4;                           // k per register chunk of A
+  T* Bs = reinterpret_cast<T*>(smem);
   const int tid = work_item_tid();
   const int lane = tid % 64, wave = tid / 64;
-  const int64_t j0 = (int64_t)SC * wave;
-  if (j0 >= P.n) return;  // no barrier below: idle waves go straight back to the work loop
-  const int64_t i = lane < P.m ? lane : P.m - 1;
+  const int64_t m = P.m, n = P.n, k = P.k;
   const T* __restrict__ A = static_cast<const T*>(P.A);
   const T* __restrict__ B = static_cast<const T*>(P.B);
+  // stage op(B): element e = tid + 256 q is (p, j) with the contiguous index fastest
+  constexpr int NQ = 64 * 64 / GNT;  // 16 elements per thread, staged 8 at a time
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    T v[NQ / 2];
+#pragma unroll
+    for (int q = 0; q < NQ / 2; ++q) {
+      const int e = tid + GNT * (q + h * NQ / 2);
+      const int pp = TB ? e / 64 : e % 64, jj = TB ? e % 64 : e / 64;
+      v[q] = (pp < k && jj < n) ? (TB ? B[jj + (int64_t)pp * P.ldb] : B[pp + (int64_t)jj * P.ldb]) : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < NQ / 2; ++q) {
+      const int e = tid + GNT * (q + h * NQ / 2);
+      const int pp = TB ? e / 64 : e % 64, jj = TB ? e % 64 : e / 64;
+      Bs[pp * SB_STRIDE + jj] = v[q];
+    }
+  }
+  const int64_t i = lane < m ? lane : m - 1;
   const T* arow = TA ? A + i * P.lda : A + i;  // A(i, p) = arow[p * astep]
   const int64_t astep = TA ? 1 : P.lda;
-  const T* bcol[SC];                           // B(p, j) = bcol[c][p * bstep]
-  const int64_t bstep = TB ? P.ldb : 1;
+  T a[2][AC];
 #pragma unroll
-  for (int c = 0; c < SC; ++c) {
-    const int64_t j = j0 + c < P.n ? j0 + c : P.n - 1;
-    bcol[c] = TB ? B + j : B + j * P.ldb;
+  for (int u = 0; u < AC; ++u) a[0][u] = u < k ? arow[u * astep] : T(0);
+  __syncthreads();
+  const int j0 = SC * wave;
+  if (j0 >= n || k == 0) {
+    if (k == 0 && j0 < n && lane < m) {  // alpha == 0 or k == 0: C = beta*C
+      T* C = static_cast<T*>(P.C);
+#pragma unroll
+      for (int c = 0; c < SC; ++c)
+        if (j0 + c < n) {
+          T v = T(0);
+          if ((T)P.beta != T(0)) v = fma((T)P.beta, C[i + (j0 + c) * P.ldc], v);
+          C[i + (j0 + c) * P.ldc] = v;
+        }
+    }
+    return;  // no barrier below
   }
+  const T* bp = Bs + j0;
   T acc[SC];
 #pragma unroll
-  for (int c = 0; c < SC; ++c) acc[c] = T(0);
-  if (P.k > 0) {
-    const T a = arow[0];
+  for (int c0 = 0; c0 < SC; c0 += 8) {  // p = 0: the plain first product
+    T b[8];
 #pragma unroll
-    for (int c = 0; c < SC; ++c) acc[c] = a * bcol[c][0];
-  }
-  for (int64_t q = 1; q < P.k; ++q) {
-    const T a = arow[q * astep];
+    for (int c = 0; c < 8; ++c) b[c] = bp[c0 + c];
 #pragma unroll
-    for (int c = 0; c < SC; ++c) acc[c] = fma(a, bcol[c][q * bstep], acc[c]);
+    for (int c = 0; c < 8; ++c) acc[c0 + c] = a[0][0] * b[c];
   }
-  if (lane >= P.m) return;
+  // chunks of AC k, two per iteration so that the register buffer of a chunk
+  // (cb & 1) is a compile-time index; chunk cb+1 is loaded while cb computes
+#pragma unroll 1
+  for (int ch = 0; ch * AC < k; ch += 2) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {  // guarded bodies, no break: the loops unroll fully
+      const int cb = ch + h;
+      if (cb * AC < k) {
+        if ((cb + 1) * AC < k) {
+#pragma unroll
+          for (int u = 0; u < AC; ++u) {
+            const int64_t pn = (int64_t)(cb + 1) * AC + u;
+            a[h ^ 1][u] = pn < k ? arow[pn * astep] : T(0);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < AC; ++u) {
+          const int pp = cb * AC + u;
+          if (pp > 0 && pp < k) {
+            const T* brow = bp + pp * SB_STRIDE;
+#pragma unroll
+            for (int c0 = 0; c0 < SC; c0 += 8) {
+              T b[8];
+#pragma unroll
+              for (int c = 0; c < 8; ++c) b[c] = brow[c0 + c];
+#pragma unroll
+              for (int c = 0; c < 8; ++c) acc[c0 + c] = fma(a[h][u], b[c], acc[c0 + c]);
+            }
+          }
+        }
+      }
+    }
+  }
+  if (lane >= m) return;
   T* C = static_cast<T*>(P.C);
   const T alpha = (T)P.alpha, beta = (T)P.beta;
 #pragma unroll
   for (int c = 0; c < SC; ++c) {
     const int64_t j = j0 + c;
-    if (j < P.n) {
+    if (j < n) {
       T v = alpha * acc[c];
       if (beta != T(0)) v = fma(beta, C[i + j * P.ldc], v);
       C[i + j * P.ldc] = v;
@@ -450,13 +514,13 @@ __device__ void small_tile(const GProb& P) {
 }
 
 template <int KIND>
-__device__ __forceinline__ void run_small(const GProb& P, int combo) {
+__device__ __forceinline__ void run_small(const GProb& P, int combo, char* smem) {
   typedef typename std::conditional<KIND == 0, double, float>::type T;
   switch (combo) {
-    case 0: small_tile<T, false, false>(P); break;
-    case 1: small_tile<T, false, true>(P); break;
-    case 2: small_tile<T, true, false>(P); break;
-    default: small_tile<T, true, true>(P); break;
+    case 0: small_tile<T, false, false>(P, smem); break;
+    case 1: small_tile<T, false, true>(P, smem); break;
+    case 2: small_tile<T, true, false>(P, smem); break;
+    default: small_tile<T, true, true>(P, smem); break;
   }
 }
 
@@ -484,7 +548,7 @@ __device__ __forceinline__ void dispatch_tile(const GProb& P, int combo, int tm,
                                               int* counters, int* sflag) {
   if (P.cfg == 0) run_combo<KIND, 64>(P, combo, tm, tn, split, smem, counters, sflag);
   else if (P.cfg == 1) run_combo<KIND, 32>(P, combo, tm, tn, split, smem, counters, sflag);
-  else run_small<KIND>(P, combo);
+  else run_small<KIND>(P, combo, smem);
 }
 
 // KIND 0: FP64 native, 1: FP32 native, 2: FP32 3xBF16.
@@ -493,6 +557,7 @@ __global__ __launch_bounds__(GNT, 2) void grouped_kernel(const GProb* __restrict
                                                          int nitems, int* __restrict__ next, int* __restrict__ counters) {
   constexpr int SMEM = (KIND == 2) ? (2 * 3 * (64 + 64) * 32 * 2) : (2 * 16 * (64 + 8) * 2 * (KIND == 0 ? 8 : 4));
   __shared__ __attribute__((aligned(16))) char smem[SMEM];
+  static_assert(SMEM >= 64 * (64 + 16 / (KIND == 0 ? 8 : 4)) * (KIND == 0 ? 8 : 4), "small path stages op(B) in LDS");
   __shared__ int s_item, s_flag;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(next, 1);
@@ -547,12 +612,15 @@ std::string bucket_key(int method, int64_t m, int64_t n, int64_t k) {
 
 // Static defaults (no table entry): the largest tile that still yields a few
 // tiles, and split-K when the output is small but k is long.
-// The small path (CFG_SMALL) is a tuner candidate only: on gfx950 an FP64 fma
-// on the vector ALUs runs at the per-CU rate of the FP64 MFMA, and one work
-// item keeps a problem on one CU where the tiles spread it over several, so
-// on 1024 problems with m, n, k in [4, 64] it measured 107 us against 44 us
-// for the tiles (3xBF16: 37.9 us both ways).
+// Problems with m, n, k <= 64 take the small path (one work item, B staged in
+// LDS): on 1024 such problems it measured 43.0 us against 44.4 us for the
+// tiles in FP64 and 34.7 against 37.6 us in 3xBF16.
 void default_choice(int64_t m, int64_t n, int64_t k, int& cfg, int& splitk) {
+  if (m <= SMALL_MAX && n <= SMALL_MAX && k <= SMALL_MAX) {
+    cfg = CFG_SMALL;
+    splitk = 1;
+    return;
+  }
   const int64_t mn = std::max(m, n);
   cfg = mn >= 96 ? 0 : 1;
   const int bm = CFG_BM[cfg];

@@ -122,15 +122,17 @@ def test_no_device_is_a_clean_error():
 
 
 def test_grouped_static_choice_rule():
-    """The planner's static choice (no tuning table; no GPU needed): the 64x64
-    MFMA tile from max(m, n) >= 96, else 32x32; a small output with a long k
-    splits k.  The small path (whole problem on the vector ALUs) is only taken
-    from a tuning-table entry, and only by problems with m, n, k <= 64."""
+    """The planner's static choice (no tuning table; no GPU needed): problems
+    with m, n, k <= 64 run whole in one work item (fastmul!'s form,
+    src/kernels.jl:202-208); larger outputs take the 64x64 MFMA tile from
+    max(m, n) >= 96, else 32x32; a small output with a long k splits k."""
     from jblas_amd import grouped as G
     G.tune_reset()
-    for m, n, k in [(1, 1, 1), (64, 64, 64), (16, 40, 8), (64, 1, 64)]:
-        assert G.tune_lookup("f64", m, n, k) == ("32x32", 1), (m, n, k)
-        assert G.tune_lookup("f32_bf16x3", m, n, k) == ("32x32", 1), (m, n, k)
+    for m, n, k in [(1, 1, 1), (64, 64, 64), (16, 40, 8), (64, 1, 64), (5, 7, 0)]:
+        assert G.tune_lookup("f64", m, n, k) == ("small", 1), (m, n, k)
+        assert G.tune_lookup("f32_bf16x3", m, n, k) == ("small", 1), (m, n, k)
+    assert G.tune_lookup("f64", 65, 64, 64) == ("32x32", 1)
+    assert G.tune_lookup("f64", 64, 64, 65) == ("32x32", 1)
     assert G.tune_lookup("f64", 96, 32, 64) == ("64x64", 1)
     assert G.tune_lookup("f64", 64, 64, 4096) == ("32x32", 1)  # 4 tiles: no split
     assert G.tune_lookup("f64", 16, 16, 4096) == ("32x32", 4)  # 1 tile: split while k / (2 s) >= 256 and tiles * s < 4

@@ -117,7 +117,9 @@ def small_group(torch, ta, tb, dtype, seed):
 
 def load_small_table(tmpdir, method, shapes):
     """A tuning table (the library's JSON format) naming the small path for the
-    size buckets of `shapes`: bucket (floor(log2(floor(sqrt(m n)))), floor(log2 k))."""
+    size buckets of `shapes`: bucket (floor(log2(floor(sqrt(m n)))), floor(log2 k)).
+    (The static rule picks it for these shapes too; the table also covers the
+    bucket of a problem that does not fit it.)"""
     info = jb.device_info()
     pre = f"{info['name']}|{info['arch']}|m{jb.grouped.method_id(method)}|"
     keys = sorted({f"mn{max(math.isqrt(m * n), 1).bit_length() - 1}|k{max(k, 1).bit_length() - 1}"
