@@ -57,6 +57,10 @@ hipError_t launch_fill_uniform(uint64_t seed, int64_t n, double* out, hipStream_
 hipError_t launch_fill_uniform_f32(uint64_t seed, int64_t n, float* out, hipStream_t s);
 hipError_t launch_fill_wide(uint64_t seed, int64_t n, int emin, int emax, double* out, hipStream_t s);
 
+// Compute units of the runtime's device (from the device query at init);
+// launch geometry that depends on the device reads it.
+int runtime_compute_units();
+
 // Per-kernel timing (off unless jb_kernel_timing_enable(1)): HIP events
 // recorded on the launch stream around one kernel launch.  Not recorded while
 // the stream is being captured into a graph.

@@ -260,6 +260,8 @@ static bool capturing(hipStream_t s) {
   return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
 }
 
+int runtime_compute_units() { return rt().info.compute_units; }
+
 void timing_begin(hipStream_t s, const char* kernel) {
   if (!g_timing || capturing(s)) return;
   TimingRec r;

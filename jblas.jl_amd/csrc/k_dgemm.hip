@@ -442,13 +442,23 @@ static hipError_t launch_gemm_mfma_bt(const GemmArgs& g, hipStream_t s) {
 
 // 128-tiles run two blocks per CU (512 on the device at a time), 64-tiles at
 // least four and ~5 % below the 128-tile rate when both fill the device.  The
-// 128-tile grid is used when its last round of 512 is at least 90 % full
-// (measured at n = 1536..8192: 64-tiles win at 1536-3584 and 5120, 128-tiles
-// at 4096, 6144, 8192).
+// 128-tile grid is used when its last round of resident workgroups (512 for
+// FP64 on 256 CUs) is at least 90 % full (measured at n = 1536..8192:
+// 64-tiles win at 1536-3584 and 5120, 128-tiles at 4096, 6144, 8192).
+//
+// Round size of the 128-tile grid: the two workgroups per CU that the launch
+// bound guarantees, times the device's CU count.  (The occupancy query gives
+// FP32 three per CU; rounds of that size moved FP32 8192^3 to the 64-tile and
+// lost 6 %, so the rule stays on the guaranteed residency it was measured with.)
+static int64_t resident_128() { return 2 * (int64_t)std::max(runtime_compute_units(), 1); }
+
+// The 128-tile unless its last round of resident workgroups would be under
+// 90 % full; then the 64-tile (four times the tiles) fills the device better.
 template <typename T>
 static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
   const int64_t tiles128 = ((g.m + 127) / 128) * ((g.n + 127) / 128);
-  const int64_t slots = ((tiles128 + 511) / 512) * 512;
+  const int64_t res = resident_128();
+  const int64_t slots = ((tiles128 + res - 1) / res) * res;
   return (double)tiles128 >= 0.9 * (double)slots ? launch_gemm_mfma_bt<T, 128>(g, s)
                                                  : launch_gemm_mfma_bt<T, 64>(g, s);
 }
