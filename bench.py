@@ -50,7 +50,9 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def timed(torch, fn, steps):
+def timed(torch, fn, steps, warm=False):
+    if warm:
+        fn()  # first-launch costs (code-object load, workspace growth) are not steady state
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
@@ -120,8 +122,8 @@ def grouped_and_rrule(torch, jb):
     rr = {}
     for method in ("f32", "f32_bf16x3"):
         fwd = "native" if method == "f32" else "bf16x3"
-        ms_f = timed(torch, lambda: jb.gemm_("N", "N", 1.0, A, B, 0.0, Cm, method=fwd), 5)
-        ms_b = timed(torch, lambda: pullback_("N", "N", 1.0, A, B, W, dA, dB, method=method), 5)
+        ms_f = timed(torch, lambda: jb.gemm_("N", "N", 1.0, A, B, 0.0, Cm, method=fwd), 5, warm=True)
+        ms_b = timed(torch, lambda: pullback_("N", "N", 1.0, A, B, W, dA, dB, method=method), 5, warm=True)
         fl = 2.0 * M * N * K
         rr[method] = {"forward_ms": round(ms_f, 4), "pullback_ms": round(ms_b, 4),
                       "total_tflops": round(3 * fl / (ms_f + ms_b) / 1e9, 3)}
