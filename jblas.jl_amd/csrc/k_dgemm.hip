@@ -15,6 +15,8 @@
 // MFMA fragment read is 16 consecutive doubles per 16-lane group.
 #include "jb_internal.h"
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 namespace jb {
 namespace {
@@ -43,6 +45,7 @@ template <typename T, int R, bool MN_CONTIG, bool VEC>
 struct TileLoader {
   static constexpr int VW = 16 / sizeof(T);          // elements per 16-byte vector
   static constexpr int PER = R * BK / (VW * NT);     // vectors per thread
+  static_assert(PER >= 1 && PER * VW * NT == R * BK, "tile must split into whole 16-byte vectors per thread");
   typedef typename Vec16<T>::type V;
   V v[PER];
 
@@ -126,7 +129,8 @@ struct TileLoader {
 // ---------------------------------------------------------------------------
 // (1) MFMA GEMM: 128x128 block tile, 4 waves of 64x64 (4x4 MFMA blocks of
 //     16x16), or 64x64 with waves of 32x32 when the 128-tile grid would
-//     leave the device's last round mostly idle; BK=16 per LDS stage, two
+//     leave the device's last round mostly idle, or (FP64) 64x32 when even
+//     the 64-tiles are fewer than four per CU; BK=16 per LDS stage, two
 //     LDS stages; global loads are issued one full stage (128-tile) or four
 //     stages (64-tile) before their LDS store (the load latency, not the MFMA
 //     work, bounded the earlier form that loaded within the stage).  T = double uses
@@ -153,50 +157,54 @@ template <> struct Mfma<float> {
   static __device__ __forceinline__ int prow(int r, int g) { return 4 * g + r; }
 };
 
-// BT: square block tile, 128 (4 waves of 64 x 64) or 64 (4 waves of 32 x 32,
-// for problems with too few 128-tiles to fill the device).
-template <typename T, bool TA, bool TB, bool VEC, int BT>
+// BTM x BTN block tile, 4 waves of (BTM/2) x (BTN/2): 128 x 128, 64 x 64 (for
+// problems with too few 128-tiles to fill the device) or 64 x 32 (about one
+// 64-tile per CU: two 64 x 32 workgroups per CU then hide each other's
+// barrier and LDS latency).
+template <typename T, bool TA, bool TB, bool VEC, int BTM, int BTN>
 __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, int64_t K, T alpha,
                                                           const T* __restrict__ A, int64_t lda,
                                                           const T* __restrict__ B, int64_t ldb, T beta,
                                                           T* __restrict__ C, int64_t ldc, int tiles_m, int tiles_n) {
-  // LDS row stride in elements.  FP64: BT + 16 (144 or 80), so the k-rows read by
-  // lane groups g and g+1 of a ds_read_b64 start 32 banks apart (no bank
-  // conflicts); the k-contiguous operands (A 'T', B 'N') are stored with the
-  // row XOR kswz(k), without which their 8-k-row stores would hit the same
-  // banks at this stride.  FP32: BT plus DG_PAD_BYTES of padding.
+  // LDS row stride in elements.  FP64: tile extent + 16 (144, 80 or 48), so
+  // the k-rows read by lane groups g and g+1 of a ds_read_b64 start 32 banks
+  // apart (no bank conflicts); the k-contiguous operands (A 'T', B 'N') are
+  // stored with the row XOR kswz(k), without which their 8-k-row stores would
+  // hit the same banks at this stride.  FP32: extent plus DG_PAD_BYTES.
   constexpr bool F64 = sizeof(T) == 8;
-  constexpr int STRIDE = F64 ? BT + 16 : BT + DG_PAD_BYTES / (int)sizeof(T);
-  constexpr int WI = BT / 32;  // 16 x 16 MFMA blocks per wave along m and n
+  constexpr int STRIDE_A = F64 ? BTM + 16 : BTM + DG_PAD_BYTES / (int)sizeof(T);
+  constexpr int STRIDE_B = F64 ? BTN + 16 : BTN + DG_PAD_BYTES / (int)sizeof(T);
+  constexpr int WIM = BTM / 32, WIN = BTN / 32;  // 16 x 16 MFMA blocks per wave along m, n
   constexpr bool SWA = F64 && TA, SWB = F64 && !TB;
-  __shared__ __attribute__((aligned(16))) T As[2][BK * STRIDE];
-  __shared__ __attribute__((aligned(16))) T Bs[2][BK * STRIDE];
+  __shared__ __attribute__((aligned(16))) T As[2][BK * STRIDE_A];
+  __shared__ __attribute__((aligned(16))) T Bs[2][BK * STRIDE_B];
   typedef typename Mfma<T>::acc_t acc_t;
 
   const int nblocks = tiles_m * tiles_n;
   const int tile = xcd_remap(blockIdx.x, nblocks, 8);
   int tm, tn;
   tile_coords(tile, tiles_m, tiles_n, 8, tm, tn);
-  const int64_t m0 = (int64_t)tm * BT, n0 = (int64_t)tn * BT;
+  const int64_t m0 = (int64_t)tm * BTM, n0 = (int64_t)tn * BTN;
 
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-  const int wm0 = (wave % 2) * (BT / 2), wn0 = (wave / 2) * (BT / 2);
+  const int wm0 = (wave % 2) * (BTM / 2), wn0 = (wave / 2) * (BTN / 2);
   const int g = lane / 16, l16 = lane % 16;
 
   // PD register sets carry the next PD stages: a stage's loads are issued PD
-  // stages before its LDS store.  The 64-tile runs where the grid has about one
-  // tile per CU and a stage is only 16 MFMAs per wave, far shorter than a
-  // global load, so it keeps 4 stages in flight; the 128-tile keeps 1.
-  constexpr int PD = BT == 64 ? 4 : 1;
-  TileLoader<T, BT, !TA, VEC> la[PD];
-  TileLoader<T, BT, TB, VEC> lb[PD];
-  const bool fullA = m0 + BT <= M, fullB = n0 + BT <= N;
+  // stages before its LDS store.  The smaller tiles run where the grid has
+  // about one tile per CU and a stage is only 16 MFMAs per wave or fewer, far
+  // shorter than a global load, so they keep 4 stages in flight; the 128-tile
+  // keeps 1.
+  constexpr int PD = BTM == 128 ? 1 : 4;
+  TileLoader<T, BTM, !TA, VEC> la[PD];
+  TileLoader<T, BTN, TB, VEC> lb[PD];
+  const bool fullA = m0 + BTM <= M, fullB = n0 + BTN <= N;
 
-  acc_t acc[WI][WI];
+  acc_t acc[WIM][WIN];
 #pragma unroll
-  for (int i = 0; i < WI; ++i)
+  for (int i = 0; i < WIM; ++i)
 #pragma unroll
-    for (int j = 0; j < WI; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < WIN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   const int nkt = (int)((K + BK - 1) / BK);
   // Stage j travels in register set j % PD: loaded as soon as stage j - PD
@@ -209,8 +217,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
     xb.load(B, ldb, n0, k0, N, K, fullB && fk);
   };
   load_stage(la[0], lb[0], 0);
-  la[0].template store<STRIDE, SWA>(As[0]);
-  lb[0].template store<STRIDE, SWB>(Bs[0]);
+  la[0].template store<STRIDE_A, SWA>(As[0]);
+  lb[0].template store<STRIDE_B, SWB>(Bs[0]);
 #pragma unroll
   for (int j = 1; j <= PD; ++j)
     if (j < nkt) load_stage(la[j % PD], lb[j % PD], j);
@@ -218,20 +226,20 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
 
   // MFMA t of a stage covers k = 4t..4t+3; lane group g supplies k = 4t+g.
   // Swizzled column (w0 + 16i + l16) ^ kswz(4t + g): with t < 4, kswz(4t + g) =
-  // 16 (t >> 1) + 8 (t & 1) + 4 (g >> 1), and w0 is a multiple of 32, so it is
-  // w0 + 16 (i ^ (t >> 1)) + (l16 ^ (8 (t & 1) + 4 (g >> 1))): a compile-time
-  // block offset plus one of two per-lane columns.
+  // 16 (t >> 1) + 8 (t & 1) + 4 (g >> 1), and w0 is a multiple of 16, so it is
+  // 16 ((w0/16 + i) ^ (t >> 1)) + (l16 ^ (8 (t & 1) + 4 (g >> 1))): a
+  // compile-time block offset (per wave) plus one of two per-lane columns.
   static_assert(BK == 16, "swizzle decomposition assumes four MFMA k-groups per stage");
   const int sw0 = l16 ^ (4 * (g >> 1)), sw1 = l16 ^ (8 + 4 * (g >> 1));
-  auto frags = [&](const T* as, const T* bs, int t, T (&a)[WI], T (&b)[WI]) {
-    const int row = (4 * t + g) * STRIDE;
+  const int bm0 = wm0 / 16, bn0 = wn0 / 16;
+  auto frags = [&](const T* as, const T* bs, int t, T (&a)[WIM], T (&b)[WIN]) {
     const int swc = (t & 1) ? sw1 : sw0;
 #pragma unroll
-    for (int i = 0; i < WI; ++i)
-      a[i] = as[row + (SWA ? wm0 + 16 * (i ^ (t >> 1)) + swc : wm0 + 16 * i + l16)];
+    for (int i = 0; i < WIM; ++i)
+      a[i] = as[(4 * t + g) * STRIDE_A + (SWA ? 16 * ((bm0 + i) ^ (t >> 1)) + swc : wm0 + 16 * i + l16)];
 #pragma unroll
-    for (int j = 0; j < WI; ++j)
-      b[j] = bs[row + (SWB ? wn0 + 16 * (j ^ (t >> 1)) + swc : wn0 + 16 * j + l16)];
+    for (int j = 0; j < WIN; ++j)
+      b[j] = bs[(4 * t + g) * STRIDE_B + (SWB ? 16 * ((bn0 + j) ^ (t >> 1)) + swc : wn0 + 16 * j + l16)];
   };
 
   for (int kt0 = 0; kt0 < nkt; kt0 += PD)
@@ -243,7 +251,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
     const int cur = kt & 1;
     const T* as = As[cur];
     const T* bs = Bs[cur];
-    T a[2][WI], b[2][WI];
+    T a[2][WIM], b[2][WIN];
     frags(as, bs, 0, a[0], b[0]);
 #pragma unroll
     for (int t = 0; t < BK / 4; ++t) {
@@ -254,35 +262,35 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
       // its four k values in lane-group order, so each dot product is one fma chain in
       // increasing k, as in the reference recurrence.
 #pragma unroll
-      for (int i = 0; i < WI; ++i)
+      for (int i = 0; i < WIM; ++i)
 #pragma unroll
-        for (int j = 0; j < WI; ++j) acc[i][j] = Mfma<T>::mma(b[t & 1][j], a[t & 1][i], acc[i][j]);
+        for (int j = 0; j < WIN; ++j) acc[i][j] = Mfma<T>::mma(b[t & 1][j], a[t & 1][i], acc[i][j]);
       // The next stage goes to the other LDS buffer (last read before the
       // previous barrier) one MFMA group before the end, so its writes
       // complete under the last group instead of in front of the barrier;
       // the registers then take the stage after it.
       if (t == BK / 4 - 2 && kt + 1 < nkt) {
-        la[nset].template store<STRIDE, SWA>(As[cur ^ 1]);
-        lb[nset].template store<STRIDE, SWB>(Bs[cur ^ 1]);
+        la[nset].template store<STRIDE_A, SWA>(As[cur ^ 1]);
+        lb[nset].template store<STRIDE_B, SWB>(Bs[cur ^ 1]);
         if (kt + 1 + PD < nkt) load_stage(la[nset], lb[nset], kt + 1 + PD);
       }
     }
     __syncthreads();
   }
 
-  if (m0 + BT <= M && n0 + BT <= N) {
+  if (m0 + BTM <= M && n0 + BTN <= N) {
     // Interior tile: for each 16-row strip, the beta*C loads are issued
     // together, then the results stored.
 #pragma unroll
-    for (int i = 0; i < WI; ++i) {
+    for (int i = 0; i < WIM; ++i) {
       T* Ci = C + (m0 + wm0 + 16 * i + l16);
-      T cv[WI][4];
+      T cv[WIN][4];
 #pragma unroll
-      for (int j = 0; j < WI; ++j)
+      for (int j = 0; j < WIN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) cv[j][r] = beta != T(0) ? Ci[(n0 + wn0 + 16 * j + Mfma<T>::prow(r, g)) * ldc] : T(0);
 #pragma unroll
-      for (int j = 0; j < WI; ++j)
+      for (int j = 0; j < WIN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           T c = alpha * acc[i][j][r];
@@ -293,10 +301,10 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
     return;
   }
 #pragma unroll
-  for (int i = 0; i < WI; ++i) {
+  for (int i = 0; i < WIM; ++i) {
     const int64_t m = m0 + wm0 + 16 * i + l16;
 #pragma unroll
-    for (int j = 0; j < WI; ++j)
+    for (int j = 0; j < WIN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t n = n0 + wn0 + 16 * j + Mfma<T>::prow(r, g);
@@ -409,9 +417,9 @@ bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 
 
 }  // namespace
 
-template <typename T, int BT>
+template <typename T, int BTM, int BTN>
 static hipError_t launch_gemm_mfma_bt(const GemmArgs& g, hipStream_t s) {
-  const int tiles_m = (int)((g.m + BT - 1) / BT), tiles_n = (int)((g.n + BT - 1) / BT);
+  const int tiles_m = (int)((g.m + BTM - 1) / BTM), tiles_n = (int)((g.n + BTN - 1) / BTN);
   const int nblocks = tiles_m * tiles_n;
   constexpr int VW = 16 / sizeof(T);
   const bool vec = aligned16(g.A) && aligned16(g.B) && (g.lda % VW == 0) && (g.ldb % VW == 0);
@@ -421,7 +429,7 @@ static hipError_t launch_gemm_mfma_bt(const GemmArgs& g, hipStream_t s) {
   const T alpha = (T)g.alpha, beta = (T)g.beta;
   const char* tname = sizeof(T) == 8 ? "dgemm_mfma" : "sgemm_mfma";
 #define JB_LAUNCH(TA, TB, V)                                                                           \
-  hipLaunchKernelGGL((gemm_mfma_kernel<T, TA, TB, V, BT>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, alpha, \
+  hipLaunchKernelGGL((gemm_mfma_kernel<T, TA, TB, V, BTM, BTN>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, alpha, \
                      A, g.lda, B, g.ldb, beta, C, g.ldc, tiles_m, tiles_n)
   timing_begin(s, tname);
   if (vec) {
@@ -440,8 +448,9 @@ static hipError_t launch_gemm_mfma_bt(const GemmArgs& g, hipStream_t s) {
   return hipGetLastError();
 }
 
-// 128-tiles run two blocks per CU (512 on the device at a time), 64-tiles at
-// least four and ~5 % below the 128-tile rate when both fill the device.  The
+// Tile choice.  128-tiles run two blocks per CU (512 on the device at a time),
+// 64-tiles at least four and ~5 % below the 128-tile rate when both fill the
+// device; FP64 problems too small for four 64-tiles per CU take 64 x 32.  The
 // 128-tile grid is used when its last round of resident workgroups (512 for
 // FP64 on 256 CUs) is at least 90 % full (measured at n = 1536..8192:
 // 64-tiles win at 1536-3584 and 5120, 128-tiles at 4096, 6144, 8192).
@@ -459,8 +468,22 @@ static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
   const int64_t tiles128 = ((g.m + 127) / 128) * ((g.n + 127) / 128);
   const int64_t res = resident_128();
   const int64_t slots = ((tiles128 + res - 1) / res) * res;
-  return (double)tiles128 >= 0.9 * (double)slots ? launch_gemm_mfma_bt<T, 128>(g, s)
-                                                 : launch_gemm_mfma_bt<T, 64>(g, s);
+  const char* force = getenv("JB_GEMM_TILE");  // testing: pin one variant ("128", "64", "64x32")
+  if (force && *force) {
+    if (!strcmp(force, "128")) return launch_gemm_mfma_bt<T, 128, 128>(g, s);
+    if (!strcmp(force, "64")) return launch_gemm_mfma_bt<T, 64, 64>(g, s);
+    if constexpr (sizeof(T) == 8)  // FP32 64 x 32 would leave threads without a whole vector
+      if (!strcmp(force, "64x32")) return launch_gemm_mfma_bt<T, 64, 32>(g, s);
+  }
+  if ((double)tiles128 >= 0.9 * (double)slots) return launch_gemm_mfma_bt<T, 128, 128>(g, s);
+  // FP64 with fewer than four 64-tiles per CU: 64 x 32 tiles (two per CU hide
+  // each other's barrier and LDS waits; measured +40 % at 512^3, +10 % at
+  // 1536^3, +3 % at 1024^3, even at 2048^3 where the 64-tile grid has 4 per CU)
+  if constexpr (sizeof(T) == 8) {
+    const int64_t tiles64 = ((g.m + 63) / 64) * ((g.n + 63) / 64);
+    if (tiles64 < 4 * (int64_t)std::max(runtime_compute_units(), 1)) return launch_gemm_mfma_bt<T, 64, 32>(g, s);
+  }
+  return launch_gemm_mfma_bt<T, 64, 64>(g, s);
 }
 
 hipError_t launch_dgemm_mfma(const GemmArgs& g, hipStream_t s) { return launch_gemm_mfma<double>(g, s); }

@@ -130,6 +130,26 @@ def test_fastmul_sign_of_zero_and_neighbours(gpu):
     assert np.all(buf[M * P:].cpu().numpy() == 7.0)
 
 
+@pytest.mark.parametrize("tile", ["128", "64", "64x32"])
+def test_native_tile_variants_bitexact(gpu, tile, monkeypatch):
+    """Each MFMA tile variant (JB_GEMM_TILE pins it; the size rule picks it in
+    normal use) is bit-identical to the recurrence, with fringes in m, n and k,
+    every op and alpha/beta."""
+    monkeypatch.setenv("JB_GEMM_TILE", tile)
+    rng = np.random.default_rng(len(tile))
+    for m, n, k in [(300, 200, 100), (64, 32, 16), (129, 65, 47)]:
+        A0 = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
+        B0 = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+        C0 = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+        for ta in "NT":
+            for tb in "NT":
+                ref = O.gemm_recurrence("N", "N", 1.5, A0, B0, 0.5, C0)  # op(opnd(X, t)) == X
+                Cd = to_dev(gpu, C0)
+                jb.gemm_(ta, tb, 1.5, to_dev(gpu, opnd(A0, ta)), to_dev(gpu, opnd(B0, tb)), 0.5, Cd)
+                gpu.cuda.synchronize()
+                assert np.array_equal(to_host(Cd), ref), (tile, m, n, k, ta, tb)
+
+
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
 def test_dgemm_c2_1024(gpu, ta, tb):
     """Config: C = 1.5*op(A)*op(B) + 0.5*C, M=N=K=1024, uniform[-1,1) seeds 42/43/44."""
