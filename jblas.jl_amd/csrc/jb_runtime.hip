@@ -435,16 +435,20 @@ static int jmul_validate(int64_t M, int64_t N, int64_t P, const void* D, const v
   return JB_OK;
 }
 
+// jmul!(D, A, X) == gemm('N', 'N', 1, A, X, 0, D) on the FP64 MFMA kernel: its
+// per-element chains are the reference recurrence bit for bit (plain first
+// product, fma k-steps in increasing k, signed zeros included); N >= 1, so
+// the chain is never empty.
 int jb_jmul_dev(int64_t M, int64_t N, int64_t P, double* D, const double* A, const double* X, void* stream) {
   int rc = jmul_validate(M, N, P, D, A, X);
   if (rc) return rc;
-  return jb_dgemm_recurrence_dev('N', 'N', M, P, N, 1.0, A, M > 1 ? M : 1, X, N, 0.0, D, M > 1 ? M : 1, stream);
+  return jb_dgemm_dev('N', 'N', M, P, N, 1.0, A, M > 1 ? M : 1, X, N, 0.0, D, M > 1 ? M : 1, stream);
 }
 
 int jb_jmul(int64_t M, int64_t N, int64_t P, double* D, const double* A, const double* X) {
   int rc = jmul_validate(M, N, P, D, A, X);
   if (rc) return rc;
-  return jb_dgemm_recurrence('N', 'N', M, P, N, 1.0, A, M > 1 ? M : 1, X, N, 0.0, D, M > 1 ? M : 1);
+  return jb_dgemm('N', 'N', M, P, N, 1.0, A, M > 1 ? M : 1, X, N, 0.0, D, M > 1 ? M : 1);
 }
 
 // ---- kernel! / initkernel! (strided microkernels) --------------------------

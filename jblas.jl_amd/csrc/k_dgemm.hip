@@ -41,8 +41,11 @@ __device__ __forceinline__ int kswz(int k) { return 4 * ((k >> 1) & 7); }
 //   MN_CONTIG: element (r, kk) at P[(r0+r) + (k0+kk)*ld]  (A 'N', B 'T')
 //   otherwise: element (r, kk) at P[(k0+kk) + (r0+r)*ld]  (A 'T', B 'N')
 // VEC: 16-byte loads are legal (base 16B aligned, ld a multiple of 16/sizeof(T)).
-template <typename T, int R, bool MN_CONTIG, bool VEC>
+// NEGZ: elements outside the operand read as -0.0 instead of +0.0 (see the
+// FP64 MFMA kernel: a -0 padding product leaves every chain value unchanged).
+template <typename T, int R, bool MN_CONTIG, bool VEC, bool NEGZ = false>
 struct TileLoader {
+  static constexpr T PADV = NEGZ ? T(-0.0) : T(0);
   static constexpr int VW = 16 / sizeof(T);          // elements per 16-byte vector
   static constexpr int PER = R * BK / (VW * NT);     // vectors per thread
   static_assert(PER >= 1 && PER * VW * NT == R * BK, "tile must split into whole 16-byte vectors per thread");
@@ -78,7 +81,7 @@ struct TileLoader {
         } else {
           const bool kok = (k0 + kk) < K;
 #pragma unroll
-          for (int e = 0; e < VW; ++e) v[i][e] = (kok && r0 + r + e < Rtot) ? src[e] : T(0);
+          for (int e = 0; e < VW; ++e) v[i][e] = (kok && r0 + r + e < Rtot) ? src[e] : PADV;
         }
       }
     } else {
@@ -96,7 +99,7 @@ struct TileLoader {
         } else {
           const bool rok = (r0 + r) < Rtot;
 #pragma unroll
-          for (int e = 0; e < VW; ++e) v[i][e] = (rok && k0 + kk + e < K) ? src[e] : T(0);
+          for (int e = 0; e < VW; ++e) v[i][e] = (rok && k0 + kk + e < K) ? src[e] : PADV;
         }
       }
     }
@@ -196,15 +199,23 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
   // shorter than a global load, so they keep 4 stages in flight; the 128-tile
   // keeps 1.
   constexpr int PD = BTM == 128 ? 1 : 4;
-  TileLoader<T, BTM, !TA, VEC> la[PD];
+  TileLoader<T, BTM, !TA, VEC, F64> la[PD];  // FP64: A's k-padding is -0.0 (below)
   TileLoader<T, BTN, TB, VEC> lb[PD];
   const bool fullA = m0 + BTM <= M, fullB = n0 + BTN <= N;
 
+  // FP64 chains start from -0.0: the first MFMA step fma(a, x, -0.0) is then
+  // exactly fl(a * x) for every input, the plain first product of the
+  // reference recurrence (src/gemm.jl:86), signed zeros included (a start of
+  // +0.0 would turn an all -0 chain into +0).  The k-steps past K that the
+  // last 16-k stage pads must leave the chain unchanged too: A pads with -0.0
+  // and B with +0.0, so each padding product is -0 and x + (-0) == x for
+  // every x, zeros included.  Callers route k == 0 elsewhere.
+  constexpr T acc0 = F64 ? T(-0.0) : T(0);
   acc_t acc[WIM][WIN];
 #pragma unroll
   for (int i = 0; i < WIM; ++i)
 #pragma unroll
-    for (int j = 0; j < WIN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < WIN; ++j) acc[i][j] = acc_t{acc0, acc0, acc0, acc0};
 
   const int nkt = (int)((K + BK - 1) / BK);
   // Stage j travels in register set j % PD: loaded as soon as stage j - PD

@@ -94,6 +94,29 @@ def test_jmul_sign_of_zero(gpu):
     D = np.ones((1, 1), order="F")
     jb.jmul_(D, np.array([[-1.0]]), np.array([[0.0]]))
     assert D[0, 0] == 0.0 and math.copysign(1.0, D[0, 0]) < 0  # acc_1 = fl(a*x), src/gemm.jl:86
+    # a chain of -0 products stays -0 (fma(-1, 0, -0) = -0); +0 then -0 gives +0
+    for A, sign in [(np.array([[-1.0, -2.0]]), -1.0), (np.array([[1.0, -1.0]]), 1.0), (np.array([[-1.0, 3.0]]), 1.0)]:
+        X = np.zeros((2, 1), order="F")
+        for f in (lambda D: jb.jmul_(D, A, X), lambda D: jb.gemm_("N", "N", 1.0, A, X, 0.0, D)):
+            D = np.full((1, 1), np.nan, order="F")
+            f(D)
+            assert D[0, 0] == 0.0 and math.copysign(1.0, D[0, 0]) == sign, (A, sign)
+
+
+def test_jmul_bits_equal_recurrence_kernel_with_signed_zeros(gpu):
+    """jmul! (FP64 MFMA chains started at -0.0) against the sequential-fma VALU
+    kernel on inputs full of +-0 and small integers: every bit equal, the sign
+    of every zero included (array_equal would not see it)."""
+    rng = np.random.default_rng(77)
+    for M, N, P in [(100, 37, 70), (256, 256, 256), (65, 1, 33)]:
+        A = np.asfortranarray(rng.choice([-2.0, -1.0, -0.0, 0.0, 1.0, 3.0], (M, N), p=[.1, .1, .3, .3, .1, .1]))
+        X = np.asfortranarray(rng.choice([-1.0, -0.0, 0.0, 2.0], (N, P), p=[.2, .3, .3, .2]))
+        D = np.full((M, P), np.nan, order="F")
+        jb.jmul_(D, A, X)
+        R = np.full((M, P), np.nan, order="F")
+        jb.gemm_("N", "N", 1.0, A, X, 0.0, R, method="recurrence")
+        assert np.array_equal(D.view(np.int64), R.view(np.int64)), (M, N, P)
+        assert np.array_equal(D.view(np.int64), O.jmul(A, X).view(np.int64)), (M, N, P)
 
 
 @pytest.mark.parametrize("M,N,P", [(1, 1, 1), (7, 1, 9), (13, 12, 12), (40, 256, 5), (64, 64, 64),
