@@ -24,9 +24,9 @@ struct GemmArgs {
 hipError_t launch_dgemm_mfma(const GemmArgs& g, hipStream_t s);
 // Sequential-fma FP64 GEMM (bit-exact jmul! recurrence).
 hipError_t launch_dgemm_recurrence(const GemmArgs& g, hipStream_t s);
-// kernel! (D += A*X, 'N','N'): each chain starts from C and every step is an
-// fma; alpha/beta unused.
-hipError_t launch_dgemm_kernel_acc(const GemmArgs& g, hipStream_t s);
+// kernel! (D += A*X) on the FP64 MFMA kernel: each chain starts from C and
+// every step is an fma; alpha/beta unused.
+hipError_t launch_dgemm_mfma_acc(const GemmArgs& g, hipStream_t s);
 // C = beta*C (alpha == 0 or k == 0 quick path); beta == 0 writes zeros.
 hipError_t launch_scale_f64(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipStream_t s);
 hipError_t launch_scale_f32(int64_t m, int64_t n, float beta, float* C, int64_t ldc, hipStream_t s);

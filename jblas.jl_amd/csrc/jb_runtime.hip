@@ -482,9 +482,15 @@ static int kernel_validate(bool init, int64_t M, int64_t N, int64_t P, const voi
   return JB_OK;
 }
 
+// kernel! / initkernel! on the FP64 MFMA kernel, whose chains are the
+// reference's bit for bit: initkernel! is D = A*X (chains from -0.0, i.e. a
+// plain first product), kernel! starts each chain from D.
 static int kernel_dev_locked(const GemmArgs& g, bool init, hipStream_t s) {
   if (g.m == 0 || g.n == 0 || (!init && g.k == 0)) return JB_OK;
-  const hipError_t e = init ? launch_dgemm_recurrence(g, s) : launch_dgemm_kernel_acc(g, s);
+  GemmArgs h = g;
+  h.alpha = 1.0;
+  h.beta = 0.0;
+  const hipError_t e = init ? launch_dgemm_mfma(h, s) : launch_dgemm_mfma_acc(h, s);
   return e == hipSuccess ? JB_OK : hip_fail(e, init ? "initkernel launch" : "kernel launch");
 }
 

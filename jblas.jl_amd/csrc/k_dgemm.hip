@@ -168,7 +168,8 @@ template <typename T, bool TA, bool TB, bool VEC, int BTM, int BTN>
 __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, int64_t K, T alpha,
                                                           const T* __restrict__ A, int64_t lda,
                                                           const T* __restrict__ B, int64_t ldb, T beta,
-                                                          T* __restrict__ C, int64_t ldc, int tiles_m, int tiles_n) {
+                                                          T* __restrict__ C, int64_t ldc, int tiles_m, int tiles_n,
+                                                          int acc_from_c) {
   // LDS row stride in elements.  FP64: tile extent + 16 (144, 80 or 48), so
   // the k-rows read by lane groups g and g+1 of a ds_read_b64 start 32 banks
   // apart (no bank conflicts); the k-contiguous operands (A 'T', B 'N') are
@@ -210,12 +211,28 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
   // last 16-k stage pads must leave the chain unchanged too: A pads with -0.0
   // and B with +0.0, so each padding product is -0 and x + (-0) == x for
   // every x, zeros included.  Callers route k == 0 elsewhere.
+  // acc_from_c (kernel! semantics, src/kernels.jl:225-233): each chain starts
+  // from the element of C instead, every k-step an fma; the caller passes
+  // alpha = 1, beta = 0, so the epilogue stores the chain value itself.
   constexpr T acc0 = F64 ? T(-0.0) : T(0);
   acc_t acc[WIM][WIN];
 #pragma unroll
   for (int i = 0; i < WIM; ++i)
 #pragma unroll
     for (int j = 0; j < WIN; ++j) acc[i][j] = acc_t{acc0, acc0, acc0, acc0};
+  if (acc_from_c) {
+#pragma unroll
+    for (int i = 0; i < WIM; ++i) {
+      const int64_t m = m0 + wm0 + 16 * i + l16;
+#pragma unroll
+      for (int j = 0; j < WIN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t n = n0 + wn0 + 16 * j + Mfma<T>::prow(r, g);
+          if (m < M && n < N) acc[i][j][r] = C[m + n * ldc];
+        }
+    }
+  }
 
   const int nkt = (int)((K + BK - 1) / BK);
   // Stage j travels in register set j % PD: loaded as soon as stage j - PD
@@ -335,9 +352,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
 constexpr int RB = 64;
 constexpr int RSTRIDE = RB + 2;
 
-// ACC: kernel! semantics (src/kernels.jl:225-233): each chain starts from the
-// element of C and every k-step is an fma; C = acc (alpha, beta unused).
-template <bool TA, bool TB, bool ACC>
+template <bool TA, bool TB>
 __global__ __launch_bounds__(NT, 2) void dgemm_recurrence_kernel(int64_t M, int64_t N, int64_t K, double alpha,
                                                                  const double* __restrict__ A, int64_t lda,
                                                                  const double* __restrict__ B, int64_t ldb,
@@ -360,7 +375,7 @@ __global__ __launch_bounds__(NT, 2) void dgemm_recurrence_kernel(int64_t M, int6
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int64_t m = m0 + tx + 16 * i, n = n0 + ty + 16 * j;
-      acc[i][j] = (ACC && m < M && n < N) ? C[m + n * ldc] : 0.0;
+      acc[i][j] = 0.0;
     }
 
   for (int64_t k0 = 0; k0 < K; k0 += BK) {
@@ -372,7 +387,7 @@ __global__ __launch_bounds__(NT, 2) void dgemm_recurrence_kernel(int64_t M, int6
     __syncthreads();
     const int kmax = (int)min<int64_t>(BK, K - k0);
     int kk = 0;
-    if (!ACC && k0 == 0) {  // acc_1 = fl(a_1 * x_1)   (src/gemm.jl:86)
+    if (k0 == 0) {  // acc_1 = fl(a_1 * x_1)   (src/gemm.jl:86)
       double a[4], b[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) a[i] = As[tx + 16 * i];
@@ -402,13 +417,9 @@ __global__ __launch_bounds__(NT, 2) void dgemm_recurrence_kernel(int64_t M, int6
     for (int j = 0; j < 4; ++j) {
       const int64_t m = m0 + tx + 16 * i, n = n0 + ty + 16 * j;
       if (m < M && n < N) {
-        if (ACC) {
-          C[m + n * ldc] = acc[i][j];
-        } else {
-          double c = alpha * acc[i][j];
-          if (beta != 0.0) c = fma(beta, C[m + n * ldc], c);
-          C[m + n * ldc] = c;
-        }
+        double c = alpha * acc[i][j];
+        if (beta != 0.0) c = fma(beta, C[m + n * ldc], c);
+        C[m + n * ldc] = c;
       }
     }
 }
@@ -429,7 +440,7 @@ bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 
 }  // namespace
 
 template <typename T, int BTM, int BTN>
-static hipError_t launch_gemm_mfma_bt(const GemmArgs& g, hipStream_t s) {
+static hipError_t launch_gemm_mfma_bt(const GemmArgs& g, hipStream_t s, int acc_from_c) {
   const int tiles_m = (int)((g.m + BTM - 1) / BTM), tiles_n = (int)((g.n + BTN - 1) / BTN);
   const int nblocks = tiles_m * tiles_n;
   constexpr int VW = 16 / sizeof(T);
@@ -441,7 +452,7 @@ static hipError_t launch_gemm_mfma_bt(const GemmArgs& g, hipStream_t s) {
   const char* tname = sizeof(T) == 8 ? "dgemm_mfma" : "sgemm_mfma";
 #define JB_LAUNCH(TA, TB, V)                                                                           \
   hipLaunchKernelGGL((gemm_mfma_kernel<T, TA, TB, V, BTM, BTN>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, alpha, \
-                     A, g.lda, B, g.ldb, beta, C, g.ldc, tiles_m, tiles_n)
+                     A, g.lda, B, g.ldb, beta, C, g.ldc, tiles_m, tiles_n, acc_from_c)
   timing_begin(s, tname);
   if (vec) {
     if (!g.ta && !g.tb) JB_LAUNCH(false, false, true);
@@ -475,59 +486,62 @@ static int64_t resident_128() { return 2 * (int64_t)std::max(runtime_compute_uni
 // The 128-tile unless its last round of resident workgroups would be under
 // 90 % full; then the 64-tile (four times the tiles) fills the device better.
 template <typename T>
-static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
+static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s, int acc_from_c = 0) {
   const int64_t tiles128 = ((g.m + 127) / 128) * ((g.n + 127) / 128);
   const int64_t res = resident_128();
   const int64_t slots = ((tiles128 + res - 1) / res) * res;
   const char* force = getenv("JB_GEMM_TILE");  // testing: pin one variant ("128", "64", "64x32")
   if (force && *force) {
-    if (!strcmp(force, "128")) return launch_gemm_mfma_bt<T, 128, 128>(g, s);
-    if (!strcmp(force, "64")) return launch_gemm_mfma_bt<T, 64, 64>(g, s);
+    if (!strcmp(force, "128")) return launch_gemm_mfma_bt<T, 128, 128>(g, s, acc_from_c);
+    if (!strcmp(force, "64")) return launch_gemm_mfma_bt<T, 64, 64>(g, s, acc_from_c);
     if constexpr (sizeof(T) == 8)  // FP32 64 x 32 would leave threads without a whole vector
-      if (!strcmp(force, "64x32")) return launch_gemm_mfma_bt<T, 64, 32>(g, s);
+      if (!strcmp(force, "64x32")) return launch_gemm_mfma_bt<T, 64, 32>(g, s, acc_from_c);
   }
-  if ((double)tiles128 >= 0.9 * (double)slots) return launch_gemm_mfma_bt<T, 128, 128>(g, s);
+  if ((double)tiles128 >= 0.9 * (double)slots) return launch_gemm_mfma_bt<T, 128, 128>(g, s, acc_from_c);
   // FP64 with fewer than four 64-tiles per CU: 64 x 32 tiles (two per CU hide
   // each other's barrier and LDS waits; measured +40 % at 512^3, +10 % at
   // 1536^3, +3 % at 1024^3, even at 2048^3 where the 64-tile grid has 4 per CU)
   if constexpr (sizeof(T) == 8) {
     const int64_t tiles64 = ((g.m + 63) / 64) * ((g.n + 63) / 64);
-    if (tiles64 < 4 * (int64_t)std::max(runtime_compute_units(), 1)) return launch_gemm_mfma_bt<T, 64, 32>(g, s);
+    if (tiles64 < 4 * (int64_t)std::max(runtime_compute_units(), 1)) return launch_gemm_mfma_bt<T, 64, 32>(g, s, acc_from_c);
   }
-  return launch_gemm_mfma_bt<T, 64, 64>(g, s);
+  return launch_gemm_mfma_bt<T, 64, 64>(g, s, acc_from_c);
 }
 
 hipError_t launch_dgemm_mfma(const GemmArgs& g, hipStream_t s) { return launch_gemm_mfma<double>(g, s); }
+// kernel!: C = C + A*X with every chain starting from C (g.alpha, g.beta ignored)
+hipError_t launch_dgemm_mfma_acc(const GemmArgs& g, hipStream_t s) {
+  GemmArgs h = g;
+  h.alpha = 1.0;
+  h.beta = 0.0;
+  return launch_gemm_mfma<double>(h, s, 1);
+}
 hipError_t launch_sgemm_mfma(const GemmArgs& g, hipStream_t s) { return launch_gemm_mfma<float>(g, s); }
 
-static hipError_t launch_recurrence(const GemmArgs& g, bool acc, hipStream_t s) {
+hipError_t launch_dgemm_recurrence(const GemmArgs& g, hipStream_t s) {
   const int tiles_m = (int)((g.m + RB - 1) / RB), tiles_n = (int)((g.n + RB - 1) / RB);
   const int nblocks = tiles_m * tiles_n;
   auto A = static_cast<const double*>(g.A);
   auto B = static_cast<const double*>(g.B);
   auto C = static_cast<double*>(g.C);
-#define JB_LAUNCH(TA, TB, AC)                                                                                \
-  hipLaunchKernelGGL((dgemm_recurrence_kernel<TA, TB, AC>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, g.alpha, \
+#define JB_LAUNCH(TA, TB)                                                                                 \
+  hipLaunchKernelGGL((dgemm_recurrence_kernel<TA, TB>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, g.alpha, \
                      A, g.lda, B, g.ldb, g.beta, C, g.ldc, tiles_m, tiles_n)
   timing_begin(s, "dgemm_recurrence");
-  if (acc) {
-    JB_LAUNCH(false, false, true);  // kernel! takes A and X as stored
-  } else if (!g.ta && !g.tb) {
-    JB_LAUNCH(false, false, false);
+  if (!g.ta && !g.tb) {
+    JB_LAUNCH(false, false);
   } else if (!g.ta && g.tb) {
-    JB_LAUNCH(false, true, false);
+    JB_LAUNCH(false, true);
   } else if (g.ta && !g.tb) {
-    JB_LAUNCH(true, false, false);
+    JB_LAUNCH(true, false);
   } else {
-    JB_LAUNCH(true, true, false);
+    JB_LAUNCH(true, true);
   }
 #undef JB_LAUNCH
   timing_end(s, "dgemm_recurrence");
   return hipGetLastError();
 }
 
-hipError_t launch_dgemm_recurrence(const GemmArgs& g, hipStream_t s) { return launch_recurrence(g, false, s); }
-hipError_t launch_dgemm_kernel_acc(const GemmArgs& g, hipStream_t s) { return launch_recurrence(g, true, s); }
 
 hipError_t launch_scale_f64(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipStream_t s) {
   if (m == 0 || n == 0) return hipSuccess;

@@ -230,6 +230,30 @@ def test_dgemm_c2_large_sampled(gpu, n, ta, tb):
     torch.cuda.empty_cache()
 
 
+def test_kernel_initkernel_bits_with_signed_zeros(gpu):
+    """kernel! (chains from D) and initkernel! (plain first product) on the
+    FP64 MFMA kernel against the oracle, bitwise (int64 views), on inputs full
+    of +-0 -- strided views, k not a multiple of the 16-k stage, and an N = 1
+    chain whose only product is -0 on a D of -0 (stays -0) or +0 (stays +0)."""
+    rng = np.random.default_rng(5)
+    big = np.asfortranarray(rng.choice([-2.0, -1.0, -0.0, 0.0, 1.0], (300, 300), p=[.15, .15, .3, .3, .1]))
+    Dbig = np.asfortranarray(rng.choice([-0.0, 0.0, 1.0], (301, 90), p=[.4, .4, .2]))
+    for (ma, ka, kb) in [(100, 37, 70), (64, 16, 32), (33, 1, 5)]:
+        A = big[10:10 + ma, 5:5 + ka]
+        X = big[20:20 + ka, 100:100 + kb]
+        D0 = Dbig[1:1 + ma, 3:3 + kb]
+        Dw = np.asfortranarray(Dbig.copy())
+        jb.kernel_(Dw[1:1 + ma, 3:3 + kb], A, X)
+        assert np.array_equal(Dw[1:1 + ma, 3:3 + kb].view(np.int64), O.kernel_acc(D0, A, X).view(np.int64)), (ma, ka, kb)
+        Di = np.asfortranarray(np.full((301, 90), np.nan))
+        jb.initkernel_(Di[1:1 + ma, 3:3 + kb], A, X)
+        assert np.array_equal(Di[1:1 + ma, 3:3 + kb].view(np.int64), O.jmul(A, X).view(np.int64)), (ma, ka, kb)
+    for d0, sign in [(-0.0, -1.0), (0.0, 1.0)]:
+        D = np.full((1, 1), d0, order="F")
+        jb.kernel_(D, np.array([[-1.0]]), np.array([[0.0]]))  # d0 + (-0)
+        assert D[0, 0] == 0.0 and math.copysign(1.0, D[0, 0]) == sign
+
+
 def test_strided_subviews(gpu):
     """kernel!/initkernel! semantics: sub-blocks with leading dimensions != rows
     (src/kernels.jl:212-275), host and device."""
