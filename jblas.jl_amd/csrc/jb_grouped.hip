@@ -77,7 +77,9 @@ __device__ __forceinline__ int work_item_tid() {
 }
 
 // RC: the R index is the contiguous one (stride 1), else the k index is.
-template <typename T, int R, int BK, bool RC>
+// NEGZ: elements outside the operand read as -0.0 (FP64 A operand: a -0
+// padding product leaves every chain value unchanged, zeros included).
+template <typename T, int R, int BK, bool RC, bool NEGZ = false>
 struct GLoader {
   static constexpr int PER = R * BK / GNT;
   T v[PER];
@@ -91,7 +93,7 @@ struct GLoader {
     for (int i = 0; i < PER; ++i) {
       const int e = tid + GNT * i;
       const int64_t gr = r0 + row_of(e), gk = k0 + k_of(e);
-      v[i] = (gr < Rtot && gk < Kend) ? (RC ? P[gr + gk * ld] : P[gk + gr * ld]) : T(0);
+      v[i] = (gr < Rtot && gk < Kend) ? (RC ? P[gr + gk * ld] : P[gk + gr * ld]) : (NEGZ ? T(-0.0) : T(0));
     }
   }
   // k-major [kk][r] (MFMA fp64/fp32 operands)
@@ -131,8 +133,13 @@ __device__ __forceinline__ void put_result(const GProb& P, int tm, int tn, int s
     const int64_t m = (int64_t)tm * BM + i, n = (int64_t)tn * BN + j;
     if (m < P.m && n < P.n) {
       T* C = static_cast<T*>(P.C);
-      T c = (T)P.alpha * v;
-      if ((T)P.beta != T(0)) c = fma((T)P.beta, C[m + n * P.ldc], c);
+      T c;
+      if (P.k == 0) {  // no products (k == 0 or alpha == 0): C = beta*C, 0 for beta == 0
+        c = (T)P.beta != T(0) ? (T)P.beta * C[m + n * P.ldc] : T(0);
+      } else {
+        c = (T)P.alpha * v;
+        if ((T)P.beta != T(0)) c = fma((T)P.beta, C[m + n * P.ldc], c);
+      }
       C[m + n * P.ldc] = c;
     }
   } else {
@@ -189,13 +196,17 @@ __device__ void native_tile(const GProb& P, int tm, int tn, int split, char* sme
   const int64_t ke = min(P.k, kb + P.kchunk);
   const T* A = static_cast<const T*>(P.A);
   const T* B = static_cast<const T*>(P.B);
-  GLoader<T, BM, BK, !TA> la(tid);  // op(A) rows contiguous when A is not transposed
+  GLoader<T, BM, BK, !TA, sizeof(T) == 8> la(tid);  // op(A) rows contiguous when A is not transposed
   GLoader<T, BN, BK, TB> lb(tid);   // op(B) columns contiguous when B is transposed
+  // FP64 chains start from -0.0 and A's k-padding is -0.0: the first step is
+  // exactly the plain product and padding products leave the chain unchanged,
+  // so each split is the reference recurrence bit for bit, zeros included.
+  constexpr T acc0 = sizeof(T) == 8 ? T(-0.0) : T(0);
   acc_t acc[IM][JN];
 #pragma unroll
   for (int i = 0; i < IM; ++i)
 #pragma unroll
-    for (int j = 0; j < JN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < JN; ++j) acc[i][j] = acc_t{acc0, acc0, acc0, acc0};
   const int nkt = (int)((ke - kb + BK - 1) / BK);
   __syncthreads();  // LDS may still be read by the previous work item
   la.load(A, P.lda, m0, kb, P.m, ke);
@@ -444,14 +455,13 @@ __device__ __forceinline__ void small_tile(const GProb& P, char* smem) {
   __syncthreads();
   const int j0 = SC * wave;
   if (j0 >= n || k == 0) {
-    if (k == 0 && j0 < n && lane < m) {  // alpha == 0 or k == 0: C = beta*C
+    if (k == 0 && j0 < n && lane < m) {  // alpha == 0 or k == 0: C = beta*C, 0 for beta == 0
       T* C = static_cast<T*>(P.C);
 #pragma unroll
       for (int c = 0; c < SC; ++c)
         if (j0 + c < n) {
-          T v = T(0);
-          if ((T)P.beta != T(0)) v = fma((T)P.beta, C[i + (j0 + c) * P.ldc], v);
-          C[i + (j0 + c) * P.ldc] = v;
+          T* cp = C + i + (j0 + c) * P.ldc;
+          *cp = (T)P.beta != T(0) ? (T)P.beta * *cp : T(0);
         }
     }
     return;  // no barrier below

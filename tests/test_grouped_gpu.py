@@ -37,6 +37,14 @@ def dev_uniform(torch, rows, cols, seed, dtype):
     return t.T  # column-major rows x cols, column-major fill of the stream
 
 
+def dev_copy(torch, a):
+    """Column-major device copy of a float64 matrix, empty shapes included."""
+    t = torch.empty((a.shape[1], a.shape[0]), dtype=torch.float64, device="cuda")
+    if a.size:
+        t.copy_(torch.from_numpy(np.ascontiguousarray(a.T)))
+    return t.T
+
+
 def make_group(torch, count, seed, dtype, ops=("N", "N"), alpha=1.0, beta=0.0):
     shapes = G.loguniform_shapes(seed, count)
     probs = []
@@ -180,6 +188,29 @@ def test_grouped_small_path_f32(gpu, method, tmp_path):
             err = np.max(np.abs(to_host(C).astype(float) - ref) / absab)
             assert err <= 2 * k * U32, (method, ta, tb, m, n, k, err)
     jb.tune_reset()
+
+
+def test_grouped_f64_bits_with_signed_zeros(gpu):
+    """FP64 grouped results equal the oracle's (chunked) recurrence in every
+    bit, the sign of zeros included (int64 views), on inputs full of +-0:
+    MFMA tiles, the small path, split-K, and problems with k == 0 or alpha == 0
+    (C = beta*C, +0 for beta == 0), in one launch."""
+    torch = gpu
+    rng = np.random.default_rng(31)
+    shapes = [(100, 70, 37, 1.5, 0.5), (40, 30, 20, 1.0, 0.0), (16, 16, 2048, -1.0, 0.0), (130, 65, 47, 1.0, 1.0),
+              (20, 10, 0, 2.0, -0.5), (25, 12, 9, 0.0, 0.0), (25, 12, 9, 0.0, -1.0)]
+    probs = []
+    for (m, n, k, alpha, beta) in shapes:
+        A = np.asfortranarray(rng.choice([-2.0, -1.0, -0.0, 0.0, 1.0], (m, k), p=[.15, .15, .3, .3, .1]))
+        B = np.asfortranarray(rng.choice([-1.0, -0.0, 0.0, 3.0], (k, n), p=[.2, .3, .3, .2]))
+        C = np.asfortranarray(rng.choice([-0.0, 0.0, 2.0], (m, n), p=[.4, .4, .2]))
+        probs.append((m, n, k, alpha, beta, A, B, C, dev_copy(torch, A), dev_copy(torch, B), dev_copy(torch, C)))
+    jb.gemm_grouped_("f64", [GemmDesc("N", "N", p[3], p[8], p[9], p[4], p[10]) for p in probs])
+    torch.cuda.synchronize()
+    for (m, n, k, alpha, beta, A, B, C, Ad, Bd, Cd) in probs:
+        _, sk = jb.tune_lookup("f64", m, n, k)
+        ref = O.gemm_recurrence_chunked("N", "N", alpha, A, B, beta, C, jb.split_chunk(k, sk))
+        assert np.array_equal(to_host(Cd).view(np.int64), ref.view(np.int64)), (m, n, k, alpha, beta, sk)
 
 
 def test_grouped_plan_graph_replays(gpu):
