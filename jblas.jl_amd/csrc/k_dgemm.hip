@@ -316,7 +316,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
 #pragma unroll
       for (int j = 0; j < WIN; ++j)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) cv[j][r] = beta != T(0) ? Ci[(n0 + wn0 + 16 * j + Mfma<T>::prow(r, g)) * ldc] : T(0);
+        for (int r = 0; r < 4; ++r)
+          cv[j][r] = beta != T(0) ? Ci[(n0 + wn0 + 16 * j + Mfma<T>::prow(r, g)) * ldc] : T(0);
 #pragma unroll
       for (int j = 0; j < WIN; ++j)
 #pragma unroll
@@ -503,7 +504,8 @@ static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s, int acc_fro
   // 1536^3, +3 % at 1024^3, even at 2048^3 where the 64-tile grid has 4 per CU)
   if constexpr (sizeof(T) == 8) {
     const int64_t tiles64 = ((g.m + 63) / 64) * ((g.n + 63) / 64);
-    if (tiles64 < 4 * (int64_t)std::max(runtime_compute_units(), 1)) return launch_gemm_mfma_bt<T, 64, 32>(g, s, acc_from_c);
+    if (tiles64 < 4 * (int64_t)std::max(runtime_compute_units(), 1))
+      return launch_gemm_mfma_bt<T, 64, 32>(g, s, acc_from_c);
   }
   return launch_gemm_mfma_bt<T, 64, 64>(g, s, acc_from_c);
 }

@@ -244,7 +244,8 @@ def test_kernel_initkernel_bits_with_signed_zeros(gpu):
         D0 = Dbig[1:1 + ma, 3:3 + kb]
         Dw = np.asfortranarray(Dbig.copy())
         jb.kernel_(Dw[1:1 + ma, 3:3 + kb], A, X)
-        assert np.array_equal(Dw[1:1 + ma, 3:3 + kb].view(np.int64), O.kernel_acc(D0, A, X).view(np.int64)), (ma, ka, kb)
+        ref = O.kernel_acc(D0, A, X)
+        assert np.array_equal(Dw[1:1 + ma, 3:3 + kb].view(np.int64), ref.view(np.int64)), (ma, ka, kb)
         Di = np.asfortranarray(np.full((301, 90), np.nan))
         jb.initkernel_(Di[1:1 + ma, 3:3 + kb], A, X)
         assert np.array_equal(Di[1:1 + ma, 3:3 + kb].view(np.int64), O.jmul(A, X).view(np.int64)), (ma, ka, kb)
