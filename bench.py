@@ -390,8 +390,14 @@ def main():
         absab = np.abs(A_rows) @ np.abs(B_cols)
 
         def err(out_t):
+            """(max rel, max |err|/(|A||B|), normwise, max rel over entries with
+            |C*| > 1e-3 max|C*|, count of entries excluded from that) on the Dot2-checked sample."""
             got = out_t.cpu().numpy().T[np.ix_(ii, jj)]
-            return (float(np.max(np.abs(got - ref) / np.abs(ref))), float(np.max(np.abs(got - ref) / absab)))
+            d = np.abs(got - ref)
+            big = np.abs(ref) > 1e-3 * np.max(np.abs(ref))
+            return (float(np.max(d / np.abs(ref))), float(np.max(d / absab)),
+                    float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+                    float(np.max(d[big] / np.abs(ref[big]))), int(np.count_nonzero(~big)))
 
         if "ozaki" in result:
             errs["ozaki"] = err(C_oz)
@@ -400,7 +406,7 @@ def main():
         if "int8" in result:
             errs["int8"] = err(C_i8)
         for row in table:
-            row["max_rel_err"], row["max_err_over_absAB"] = err(row.pop("out"))
+            row["max_rel_err"], row["max_err_over_absAB"], row["normwise_err"] = err(row.pop("out"))[:3]
             if "native" in errs:
                 row["err_ratio_vs_native"] = round(row["max_rel_err"] / errs["native"][0], 3)
         cpu = leg["cpu"]
@@ -468,6 +474,11 @@ def main():
             if "native" in errs:
                 line["emulated_int8"]["err_ratio_vs_native"] = round(errs["int8"][0] / errs["native"][0], 4)
         line["error_sample"] = "Dot2 oracle on 64 sampled rows x 64 sampled columns (4096 entries)"
+        line["error_metrics"] = {
+            name: {"componentwise_scaled": e[1], "max_rel": e[0], "normwise": e[2],
+                   "max_rel_excluding_small": e[3], "excluded_entries": e[4]}
+            for name, e in (("emulated", errs.get("ozaki")), ("native", errs.get("native")),
+                            ("emulated_int8", errs.get("int8"))) if e is not None}
     # roofline of the dominant kernel
     if oz:
         hw_flops = pairs * flops
