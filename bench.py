@@ -6,8 +6,8 @@ DGEMM, with the max relative error of each against the FP64 (Dot2) oracle.
 A step is one C = A*B at M = N = K = 8192 (column-major, NN) on inputs already
 resident in HBM, a_ij = u*2^e with u uniform in [-1,1) and e uniform in
 [-12, 12] (the emulated-DGEMM workload), computed by the split-BF16 emulated
-DGEMM with `--nslices` slices (default 7: the smallest count whose error is
-within 4x native FP64's on this workload).  `value` is that step's
+DGEMM with `--nslices` BF16 slices of 9 bits (default 6: the smallest count
+whose componentwise error is at or below native FP64's on this workload).  `value` is that step's
 throughput, 2*M*N*K / step time.  The native FP64-MFMA DGEMM is timed on the
 same inputs for the comparison, and a slice-count table (3..8) is measured
 after the timed region.
@@ -34,7 +34,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=8192)
-    ap.add_argument("--nslices", type=int, default=7)
+    ap.add_argument("--nslices", type=int, default=6, help="BF16 slices (9 bits each)")
+    ap.add_argument("--nslices-int8", type=int, default=7, help="INT8 slices (8 bits each)")
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline and the oracle check")
     ap.add_argument("--skip-table", action="store_true", help="omit the per-slice-count table")
     ap.add_argument("--only", choices=["ozaki", "native", "ozaki_int8"], default=None,
@@ -138,7 +139,7 @@ def native_and_boundary(torch, jb):
     * jmul! at 256^3 (the reference's CPU-runnable case): device-resident
       call and the host-pointer call (operands staged over PCIe);
     * the host-pointer DGEMM at 8192^3 (PCIe-inclusive: A and B copied in,
-      C copied out), native and split-BF16 s = 7."""
+      C copied out), native and split-BF16 s = 6."""
     import numpy as np
 
     out = {"c2_native_tflops": {}}
@@ -184,7 +185,7 @@ def native_and_boundary(torch, jb):
     Bh = np.asfortranarray(np.random.default_rng(2).uniform(-1, 1, (n, n)))
     Ch = np.zeros((n, n), order="F")
     host = {}
-    for meth, kw in (("native", {}), ("ozaki_s7", {"method": "ozaki", "nslices": 7})):
+    for meth, kw in (("native", {}), ("ozaki_s6", {"method": "ozaki", "nslices": 6})):
         jb.gemm_("N", "N", 1.0, Ah, Bh, 0.0, Ch, **kw)
         t0 = time.perf_counter()
         for _ in range(2):
@@ -281,9 +282,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     jb.init(0)
     info = jb.device_info()
-    n, s = args.n, args.nslices
+    n, s, s8 = args.n, args.nslices, args.nslices_int8
     flops = 2.0 * n * n * n
-    pairs = s * (s + 1) // 2
+    pairs, pairs8 = s * (s + 1) // 2, s8 * (s8 + 1) // 2
     log(f"device {info['arch']} CUs={info['compute_units']} clock={info['clock_khz'] / 1e6:.2f}GHz "
         f"peak fp64={info['peak_fp64_tflops']:.1f} bf16={info['peak_bf16_tflops']:.1f} TF")
 
@@ -308,7 +309,7 @@ def main():
         jb.gemm_("N", "N", 1.0, Ac, Bc, 0.0, Cc, method="native")
 
     def step_int8():
-        jb.gemm_("N", "N", 1.0, Ac, Bc, 0.0, Cc, method="ozaki_int8", nslices=s)
+        jb.gemm_("N", "N", 1.0, Ac, Bc, 0.0, Cc, method="ozaki_int8", nslices=s8)
 
     result = {}
     # ---- emulated DGEMM (the measured step) ----
@@ -351,7 +352,7 @@ def main():
         si_ms, si_n = _lib.kernel_time("ozaki_i8_split")
         result["int8"] = {"ms": ms_i, "tflops": flops / ms_i / 1e9, "gemm_kernel_ms": ki_ms / max(ki_n, 1),
                           "split_ms": si_ms / max(si_n, 1)}
-        log(f"ozaki_int8 s={s}: {ms_i:.3f} ms/step  {flops / ms_i / 1e9:.2f} TF  (slice-pair kernel "
+        log(f"ozaki_int8 s={s8}: {ms_i:.3f} ms/step  {flops / ms_i / 1e9:.2f} TF  (slice-pair kernel "
             f"{ki_ms / max(ki_n, 1):.3f} ms)")
         C_i8 = C.clone() if not args.skip_cpu else None
 
@@ -455,7 +456,7 @@ def main():
         "vs_baseline": None,
         "dtype": "bf16" if oz else ("f64" if nat else "int8"),
         "data": "synthetic (seeded SplitMix64 in HBM: a_ij = u*2^e, u in [-1,1), e in [-12,12])",
-        "config": {"workload": f"dgemm {n}x{n}x{n} column-major NN, split-BF16 emulated with {s} slices "
+        "config": {"workload": f"dgemm {n}x{n}x{n} column-major NN, split-BF16 emulated with {s} slices of 9 bits "
                                f"({pairs} BF16 slice-pair products, FP32 exact level sums, FP64 combine) "
                                f"vs native FP64 MFMA",
                    "m": n, "n": n, "k": n, "nslices": s, "parallelism": "single GPU"},
@@ -468,19 +469,19 @@ def main():
     if oz and nat:
         line["speedup_emulated_vs_native"] = round(nat["ms"] / oz["ms"], 4)
     if i8:
-        line["emulated_int8"] = {"method": f"ozaki_int8, {s} INT8 slices on v_mfma_i32_32x32x32_i8 (exact int32 "
-                                           f"level sums), same digit weights and FP64 combine",
+        line["emulated_int8"] = {"method": f"ozaki_int8, {s8} INT8 slices of 8 bits on v_mfma_i32_32x32x32_i8 "
+                                           f"({pairs8} slice-pair products, exact int32 level sums, FP64 combine)",
                                  "tflops": round(i8["tflops"], 3), "ms_per_step": round(i8["ms"], 4),
                                  "split_kernels_ms": round(i8["split_ms"], 4)}
         if nat:
             line["emulated_int8"]["speedup_vs_native"] = round(nat["ms"] / i8["ms"], 4)
-        hw = pairs * flops
+        hw = pairs8 * flops
         ach_i = hw / (i8["gemm_kernel_ms"] * 1e-3) / 1e12
         peak_i8 = info["peak_bf16_tflops"] * 2.0  # int8 MFMA: 8192 ops/clk/CU
-        line["emulated_int8"]["roofline"] = {"kernel": f"slice_pair_gemm_kernel<2,{s},{s + 1},1>", "bound": "mfma",
+        line["emulated_int8"]["roofline"] = {"kernel": f"slice_pair_gemm_kernel<2,{s8},{s8 + 1},1>", "bound": "mfma",
                                              "achieved": round(ach_i, 2), "peak": round(peak_i8, 1),
                                              "unit": "TOP/s", "frac": round(ach_i / peak_i8, 4),
-                                             "traffic": profiled_traffic(f"slice_pair_gemm_i8_s{s}_n{n}"),
+                                             "traffic": profiled_traffic(f"slice_pair_gemm_i8_s{s8}_n{n}"),
                                              "avg_launch_ms": round(i8["gemm_kernel_ms"], 4)}
     if errs:
         if "ozaki" in errs:

@@ -163,8 +163,11 @@ int jb_dgemm_recurrence_dev(char transA, char transB, int64_t m, int64_t n, int6
                             int64_t ldb, double beta, double* C, int64_t ldc, void* stream);
 
 /* Emulated FP64: operands split into `nslices` (1..8) exponent-scaled BF16
- * slices, slice pairs (t+u <= nslices+1) multiplied on BF16 MFMA with exact
- * FP32 accumulation, level sums combined in compensated FP64.  The 15th
+ * slices of balanced 9-bit digits (|d| <= 256, weights 2^(-8-9t) times the
+ * row/column exponent: 9*nslices-1 bits below the row/column scale), slice
+ * pairs (t+u <= nslices+1) multiplied on BF16 MFMA with exact FP32 level sums,
+ * level sums combined in compensated FP64.  Six slices (21 pair products) are
+ * as accurate as native FP64 componentwise on wide-range 8192^3 inputs.  The 15th
  * argument (nslices) is validated as argument 14 in the _dev form too. */
 int jb_dgemm_ozaki(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
                    const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
@@ -173,7 +176,7 @@ int jb_dgemm_ozaki_dev(char transA, char transB, int64_t m, int64_t n, int64_t k
                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                        double* C, int64_t ldc, int nslices, void* stream);
 
-/* Emulated FP64 on INT8 slices: the same digit weights (2^(-7-8t) times the
+/* Emulated FP64 on INT8 slices: 8-bit digit weights (2^(-7-8t) times the
  * row/column exponent) with each digit a signed byte of the operand rounded to
  * 8*nslices-1 bits below its row/column scale, slice pairs (t+u <= nslices+1)
  * on v_mfma_i32_32x32x32_i8 with exact int32 level sums (k-chunks of 8192),

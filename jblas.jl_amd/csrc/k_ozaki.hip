@@ -7,8 +7,9 @@
 //   1. absmax  (HBM-bound): per row of op(A) / column of op(B), the bit pattern
 //      of max|x| (an unsigned max of |x| bits orders finite < inf < NaN).
 //   2. split   (HBM-bound): E = frexp exponent of that max; x = ldexp(a, -E);
-//      balanced digits d_1 = rint(x*2^7), d_t = rint(r*2^8) of the running
-//      residual, |d| <= 128, each exact in BF16.  Slice planes are zero-padded
+//      balanced digits d_1 = rint(x*2^8), d_t = rint(r*2^9) of the running
+//      residual, |d| <= 256, each exact in BF16 (an integer of magnitude <= 256
+//      has at most 8 significant bits), so a slice carries 9 bits.  Slice planes are zero-padded
 //      and stored tile-contiguous: for a row block of RB rows (128 for op(A),
 //      64 for op(B)) and a k-stage of 16, the RB x 16 digits form one
 //      contiguous block (row-major, 32 bytes per row), so a GEMM stage is a
@@ -20,19 +21,21 @@
 //      earlier ones, so a step needs two new fragments).  Operand stages stream
 //      into a 3-slot LDS ring by direct global->LDS loads
 //      (global_load_lds_dwordx4), two stages ahead, one piece per step, one
-//      s_barrier per stage.  Every kc_l k-values (kc_l*(l+1) <= 1024 for the
-//      l+1 pairs of level l, so a level sum stays within 2^24 and is exact
-//      whatever the MFMA's internal order) level l is flushed into an FP64
-//      group accumulator right after its last MFMA of the stage,
-//      G_g = fma(S_l, 2^-8j, G_g) with l = 4g+j, so the flush overlaps the
-//      remaining MFMAs.  The epilogue combines the groups by
-//      TwoSum from the smallest, rescales by 2^(E_i+F_j-14) and applies
+//      s_barrier per stage.  Every kc_l k-values (kc_l*(l+1) <= 256 for the
+//      l+1 pairs of level l, products |d_t d_u| <= 2^16, so a level sum stays
+//      within 2^24 and is exact whatever the MFMA's internal order) level l is
+//      flushed into an FP64 group accumulator right after its last MFMA of the
+//      stage, G_g = fma(S_l, 2^-9j, G_g) with l = 3g+j (exact: G_g is a
+//      multiple of 2^-18 below 2^35 while k*s <= 2^19; for s > 6, l = 4g+j),
+//      so the flush overlaps the remaining MFMAs.  The epilogue combines the groups by
+//      TwoSum from the smallest, rescales by 2^(E_i+F_j-16) and applies
 //      alpha/beta.  These are the oracle restatement's operations in the same
 //      order, so the result is bit-reproducible.
-//   INT8 mode: the same digit weights with signed-byte digits (split 2c), the
-//   same LDS image (a stage of 32 int8 k has the bytes of a BF16 stage of 16),
+//   INT8 mode: signed-byte digits of 8 bits (split 2c), weights 2^(-7-8(t-1)),
+//   the same LDS image (a stage of 32 int8 k has the bytes of a BF16 stage of 16),
 //   v_mfma_i32_32x32x32_i8 with exact int32 level sums over k-chunks of 8192,
-//   the same flush and combine.
+//   the same flush into groups of four levels (G_g = fma(S_l, 2^-8j, G_g),
+//   l = 4g+j) and the same combine with the rescale 2^(E_i+F_j-14).
 //   BF16X3 mode (FP32 inputs): per-element split a = a1 + a2 + a3 (bf16 round
 //   to nearest of the running residual, exact), products t+u <= LMAX, level 2
 //   in one FP32 accumulator and the smaller levels in a second.
@@ -66,10 +69,10 @@ constexpr int QPAD_K = 64;          // k padding of the planes (split tile width
 #endif
 
 // k-chunk of 0-based level L (L + 1 slice pairs) over which its FP32 sum of
-// products of digits |d| <= 128 stays within 2^24, i.e. exact.
+// products of digits |d| <= 256 stays within 2^24, i.e. exact.
 __host__ __device__ constexpr int level_chunk(int L) {
-  int kc = 1024;
-  while (kc * (L + 1) > 1024) kc >>= 1;
+  int kc = 256;
+  while (kc * (L + 1) > 256) kc >>= 1;
   return kc;
 }
 
@@ -205,7 +208,7 @@ __global__ __launch_bounds__(256) void split_f64_kernel(const double* __restrict
     }
     double res[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) res[e] = ldexp(tile[row][kg + e], -E) * 128.0;
+    for (int e = 0; e < 8; ++e) res[e] = ldexp(tile[row][kg + e], -E) * 256.0;
     const int64_t off = plane_off(r0 + row, k0 + kg, RB, Kpad);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -213,8 +216,8 @@ __global__ __launch_bounds__(256) void split_f64_kernel(const double* __restrict
 #pragma unroll
       for (int e = 0; e < 8; e += 2) {
         if (s > 0) {
-          res[e] *= 256.0;
-          res[e + 1] *= 256.0;
+          res[e] *= 512.0;
+          res[e + 1] *= 512.0;
         }
         const double d0 = rint(res[e]), d1 = rint(res[e + 1]);
         res[e] -= d0;
@@ -231,7 +234,7 @@ __global__ __launch_bounds__(256) void split_f64_kernel(const double* __restrict
 //     the row max, plus one when max*2^-E > 126.5/128; X = rint(x * 2^(8S-1))
 //     with x = ldexp(a, -E); the digits are X's signed base-256 digits (least
 //     significant from plane S-1 up, each in [-128, 127]), so that
-//     a ~= 2^E * sum_t d_t 2^(-7-8t), the weights of the BF16 split.  A plane
+//     a ~= 2^E * sum_t d_t 2^(-7-8t).  A plane
 //     stage is 32 int8 of k per row (one v_mfma_i32_32x32x32_i8 k-step): its
 //     byte image has the shape of a BF16 stage of 16.
 // ---------------------------------------------------------------------------
@@ -342,10 +345,11 @@ __global__ __launch_bounds__(256) void split_f32_kernel(const float* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// 3. slice-pair GEMM.  MODE 0: emulated FP64 (S BF16 slices, exact FP32 level
-//    sums, FP64 groups).  MODE 1: 3xBF16 FP32 (S = 3, pairs t+u <= LMAX).
-//    MODE 2: emulated FP64 on S INT8 slices (v_mfma_i32_32x32x32_i8, exact
-//    int32 level sums, 32 k per stage, same LDS image and combine as MODE 0).
+// 3. slice-pair GEMM.  MODE 0: emulated FP64 (S BF16 slices of 9 bits, exact
+//    FP32 level sums, FP64 groups of three levels).  MODE 1: 3xBF16 FP32
+//    (S = 3, pairs t+u <= LMAX).  MODE 2: emulated FP64 on S INT8 slices of 8
+//    bits (v_mfma_i32_32x32x32_i8, exact int32 level sums, 32 k per stage, same
+//    LDS image as MODE 0, FP64 groups of four levels).
 // ---------------------------------------------------------------------------
 struct PairArgs {
   const uint16_t* pa;  // op(A) planes, row block 128
@@ -393,7 +397,10 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   constexpr int WL_MIN = NWL / NWAVES;            // wave-loads per stage of waves >= NWL % NWAVES
   constexpr int WL_MAX = (NWL + NWAVES - 1) / NWAVES;  // ... of waves < NWL % NWAVES
   constexpr int NL = (MODE == 1) ? 2 : S;         // accumulator sets (levels)
-  constexpr int NG = (S + 3) / 4;                 // FP64 groups (MODES 0, 2)
+  constexpr int DB = (MODE == 2) ? 8 : 9;         // bits per digit (level weight 2^-DB)
+  constexpr int GL = (MODE == 2 || S > 6) ? 4 : 3;  // levels per FP64 group (three: exact groups; four
+                                                     // above six levels, whose registers three would spill)
+  constexpr int NG = (S + GL - 1) / GL;           // FP64 groups (MODES 0, 2)
   // stages per exact chunk of level L: FP32 level sums within 2^24 (MODE 0:
   // level_chunk(L) k); int32 sums of 8192 k (|S_l| <= 8 * 8192 * 2^14 = 2^30)
   // in MODE 2, whose stages hold 32 k.  All powers of two.
@@ -470,10 +477,30 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   void* eC = p.C;
   asm volatile("" : "+s"(eN), "+s"(eldc), "+s"(ealpha), "+s"(ebeta), "+s"(eC));
 
+  // Flush of level j: G_g += S_j * 2^(-DB*(j%GL)), exact (S_j is an integer below 2^24).
+  auto flush = [&](const int j, const bool zero) {
+    const double wj = __builtin_ldexp(1.0, -DB * (j % GL));  // folds to a constant
+#pragma unroll
+    for (int b = 0; b < WB; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        G[j / GL][b][e] = fma((double)acc[j][b][e], wj, G[j / GL][b][e]);
+        if (zero) acc[j][b][e] = 0;
+      }
+  };
+  // "Fast" levels (MODE 0: an exact chunk of two stages; the highest levels, the
+  // ones with the most pairs) flush without a branch: stages run in pairs
+  // (phase 0, phase 1), a fast level's chunk ends with every phase-1 stage, and
+  // its flush is deferred into the next phase-0 stage, just before the level's
+  // first MFMA there, which then starts from a zero accumulator (no zeroing
+  // pass).  Per chunk end the flushes stay in increasing level order.
+  auto fast = [&](const int L) { return MODE == 0 && S <= 6 && kc_stages(L) == 2; };  // (S > 6 would spill)
+
   const int nst = (int)(p.Kpad / QBK);
   issue(0, 0);
   if (nst > 1) issue(1, 1);
-  for (int st = 0; st < nst; ++st) {
+  auto stage = [&](const int st, auto phase) {
+    constexpr int PH = decltype(phase)::value;
     // this stage landed; the next stage's loads (this wave's count) may be in flight
     if (st + 1 < nst) {
       if (wave < NWL % NWAVES) wait_vmcnt<WL_MAX>();
@@ -492,7 +519,8 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
     // Pairs run in square order: step j multiplies the new slices A_j and B_j
     // with all earlier ones (pairs with max(t, u) == j).  A step needs only two
     // new fragments, read one step ahead, so a stage starts after two LDS reads
-    // instead of all of them.  Level l = t + u is complete after step l.
+    // instead of all of them.  Level l = t + u is complete after step l; its
+    // first pair is (t, u) = ((l+1)/2, l-(l+1)/2), at step (l+1)/2.
     bf8 af[S][WB], bf[S];
 #pragma unroll
     for (int b = 0; b < WB; ++b) af[0][b] = __builtin_bit_cast(bf8, L[(wm0 + 32 * b + l32) * 2 + phl]);
@@ -516,6 +544,10 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
         const int u = q <= j ? q : j;
         if (t + u + 2 > LMAX) continue;
         const int l = (MODE != 1) ? (t + u) : (t + u == 0 ? 0 : 1);
+        // first MFMA of a fast level in a phase-0 stage: flush the chunk that ended
+        // with the previous stage (zeros at st == 0), then start from zero
+        const bool restart = PH == 0 && fast(l) && t == (l + 1) / 2 && u == l - (l + 1) / 2;
+        if (restart) flush(l, false);
         // X = op(B) plane (p -> n), Y = op(A) plane (q -> m): item r of lane l holds
         // C[m0+wm0+32b+l%32][n0+wn0+8(r/4)+4(l/32)+r%4].
 #pragma unroll
@@ -524,20 +556,22 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
             acc[l][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(__builtin_bit_cast(i4v, bf[u]),
                                                               __builtin_bit_cast(i4v, af[t][b]), acc[l][b], 0, 0, 0);
           else
-            acc[l][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bf[u], af[t][b], acc[l][b], 0, 0, 0);
+            acc[l][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bf[u], af[t][b], restart ? AccV{} : acc[l][b], 0, 0, 0);
       }
-      if (MODE != 1 && j < NL && (last_stage || (st1 & (kc_stages(j) - 1)) == 0)) {
-        // level j has received its last product of this chunk: flush it exactly
-        constexpr double kScale[4] = {1.0, 0x1.0p-8, 0x1.0p-16, 0x1.0p-24};
-#pragma unroll
-        for (int b = 0; b < WB; ++b)
-#pragma unroll
-          for (int e = 0; e < 16; ++e) {
-            G[j / 4][b][e] = fma((double)acc[j][b][e], kScale[j % 4], G[j / 4][b][e]);
-            acc[j][b][e] = 0;
-          }
-      }
+      if (MODE != 1 && j < NL && !fast(j) && (last_stage || (st1 & (kc_stages(j) - 1)) == 0))
+        flush(j, true);  // level j has received its last product of this chunk
     }
+  };
+  // nst is a multiple of 2 (the planes' k padding is a multiple of 64); the
+  // guard keeps an odd count correct too (the last chunk then ends in phase 0)
+  for (int st = 0; st < nst; st += 2) {
+    stage(st, std::integral_constant<int, 0>());
+    if (st + 1 < nst) stage(st + 1, std::integral_constant<int, 1>());
+  }
+  if (MODE == 0) {
+#pragma unroll
+    for (int j = 0; j < NL; ++j)
+      if (fast(j)) flush(j, false);  // the fast levels' last chunk
   }
 
   if (MODE != 1) {
@@ -555,10 +589,10 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
 #pragma unroll
           for (int g = NG - 1; g >= 0; --g) {
             double err;
-            two_sum(hi, ldexp(G[g][b][e], -32 * g), hi, err);
+            two_sum(hi, ldexp(G[g][b][e], -DB * GL * g), hi, err);
             lo += err;
           }
-          const double v = ldexp(hi + lo, ea + p.eB[n] - 14);
+          const double v = ldexp(hi + lo, ea + p.eB[n] - (MODE == 2 ? 14 : 16));
           double c = ealpha * v;
           if (ebeta != 0.0) c = fma(ebeta, C[m + n * eldc], c);
           C[m + n * eldc] = c;

@@ -101,16 +101,18 @@ def _check_shapes(tA, tB, a: _Operand, b: _Operand, c: _Operand):
 
 
 METHODS_F64 = ("native", "recurrence", "ozaki", "ozaki_int8")
+# slice counts whose componentwise error is at or below native FP64's (8192^3, wide-range inputs)
+DEFAULT_NSLICES = {"ozaki": 6, "ozaki_int8": 7}
 METHODS_F32 = ("native", "bf16x3")
 
 
-def gemm_(tA: str, tB: str, alpha, A, B, beta, C, *, method: str = "native", nslices: int = 7,
+def gemm_(tA: str, tB: str, alpha, A, B, beta, C, *, method: str = "native", nslices: int | None = None,
           nprod: int = 6, stream=None):
     """C = alpha*op(A)*op(B) + beta*C in place; returns C.
 
     method (Float64): "native" FP64 MFMA, "recurrence" (bit-exact jmul! order),
-    "ozaki" (split-BF16 emulation with `nslices` slices), "ozaki_int8" (the
-    same digit weights on INT8 slices and int8 MFMA).
+    "ozaki" (split-BF16 emulation with `nslices` slices of 9 bits, default 6),
+    "ozaki_int8" (INT8 slices of 8 bits on int8 MFMA, default 7).
     method (Float32): "native" FP32 MFMA, "bf16x3" (3 BF16 slices, `nprod` products).
     """
     a, b, c = _Operand(A, False, "A"), _Operand(B, False, "B"), _Operand(C, True, "C")
@@ -128,6 +130,8 @@ def gemm_(tA: str, tB: str, alpha, A, B, beta, C, *, method: str = "native", nsl
             args = common + (float(alpha), a.ptr, a.ld, b.ptr, b.ld, float(beta), c.ptr, c.ld)
         elif method in ("ozaki", "ozaki_int8"):
             name = "jb_dgemm_ozaki" + ("_i8" if method == "ozaki_int8" else "") + ("_dev" if dev else "")
+            if nslices is None:
+                nslices = DEFAULT_NSLICES[method]
             args = common + (float(alpha), a.ptr, a.ld, b.ptr, b.ld, float(beta), c.ptr, c.ld, int(nslices))
         else:
             raise ValueError(f"Float64 method must be one of {METHODS_F64}")

@@ -9,7 +9,8 @@ jBLAS.jl's GEMM entry point.  Drop-ins:
     accepts any strided `Matrix{Float64}` (the reference takes `MMatrix`).
   * `gemm!(tA, tB, α, A, B, β, C; method)` -- LinearAlgebra.BLAS.gemm!
     argument order; Float64 methods `:native` (FP64 MFMA), `:recurrence`,
-    `:ozaki` (split-BF16, keyword `nslices`), `:ozaki_int8` (INT8 slices); Float32 `:native`, `:bf16x3`.
+    `:ozaki` (split-BF16, keyword `nslices`, default 6 slices of 9 bits), `:ozaki_int8` (INT8 slices of
+    8 bits, default 7); Float32 `:native`, `:bf16x3`.
   * `fastmul!(D, A, X)` -- the same recurrence as one small-GEMM launch
     (src/kernels.jl:202-208), Float64 or Float32, dense matrices.
   * `kernel!(D, A, X)` / `initkernel!(D, A, X)` -- D += A*X (each chain
@@ -92,7 +93,8 @@ function _dims(tA, tB, A, B, C)
 end
 
 function gemm!(tA::AbstractChar, tB::AbstractChar, α::Real, A::StridedMatrix{Float64}, B::StridedMatrix{Float64},
-               β::Real, C::StridedMatrix{Float64}; method::Symbol = :native, nslices::Integer = 7)
+               β::Real, C::StridedMatrix{Float64}; method::Symbol = :native,
+               nslices::Integer = method === :ozaki_int8 ? 7 : 6)
     (colmajor(A) && colmajor(B) && colmajor(C)) || throw(ArgumentError("unit-stride columns required"))
     m, n, k = _dims(tA, tB, A, B, C)
     if method === :native

@@ -414,27 +414,32 @@ ORC_API void orc_kernel_acc_d(int64_t M, int64_t N, int64_t P, const double* A, 
  * Scheme (this build's definition; the reference has no emulated path):
  *   E_i = exponent of max_p |op(A)[i,p]| (frexp: max = f*2^E, f in [0.5,1)),
  *         0 for an all-zero row; F_j likewise for the columns of op(B).
- *   x = ldexp(a, -E_i) in (-1, 1);  r = x*2^7;  d_1 = rint(r); r -= d_1;
- *   for t >= 2: r *= 2^8; d_t = rint(r); r -= d_t.      (|d_t| <= 128)
- *   so  a ~= 2^E_i * sum_t d_t * 2^(-7-8(t-1)).
+ *   x = ldexp(a, -E_i) in (-1, 1);  r = x*2^8;  d_1 = rint(r); r -= d_1;
+ *   for t >= 2: r *= 2^9; d_t = rint(r); r -= d_t.      (|d_t| <= 256)
+ *   so  a ~= 2^E_i * sum_t d_t * 2^(-8-9(t-1)).  Every integer of magnitude
+ *   <= 256 has at most 8 significant bits and is exact in BF16, so a digit
+ *   carries 9 bits (balanced, with sign).
  *   Level sums over k-chunks of their own length kc_l:
  *     S_l = sum_{t+u=l, t,u<=s} sum_{p in chunk} dA_t[i,p]*dB_u[p,j]   (exact integers),
- *   for l = 2..s+1 (triangular truncation).  Level l has l-1 pairs and
- *   kc_l = largest power of two with kc_l * (l-1) <= 1024 (1024, 512, 256,
- *   256, 128, ...), so that every level sum of a chunk stays within 2^24 and is
- *   exact in FP32 whatever the summation order.
+ *   for l = 2..s+1 (triangular truncation).  Level l has l-1 pairs of products
+ *   |d_t d_u| <= 2^16 and kc_l = largest power of two with kc_l * (l-1) <= 256
+ *   (256, 128, 64, 64, 32, ...), so that every level sum of a chunk stays
+ *   within 2^24 and is exact in FP32 whatever the summation order.
  *   At every multiple of the shortest chunk and at k, in increasing l, each
- *   level whose chunk ends there is flushed:  g = (l-2)/4, j = (l-2)%4,
- *     G_g = fma((double)S_l, 2^(-8j), G_g), S_l = 0     (groups of four levels).
- *   v = compensated sum over g from the smallest: w_g = ldexp(G_g, -32g);
+ *   level whose chunk ends there is flushed:  g = (l-2)/3, j = (l-2)%3,
+ *     G_g = fma((double)S_l, 2^(-9j), G_g), S_l = 0     (groups of three levels:
+ *   G_g is a multiple of 2^-18 and, when k*s <= 2^19, below 2^35, so that every
+ *   flush is then exact; for s > 6 the groups hold four levels, g = (l-2)/4,
+ *   j = (l-2)%4, and w_g below is ldexp(G_g, -36g)).
+ *   v = compensated sum over g from the smallest: w_g = ldexp(G_g, -27g);
  *       (hi, e) = TwoSum(hi, w_g), lo += e;  v = hi + lo;
- *   acc = ldexp(v, E_i + F_j - 14);  c = alpha*acc;  c = fma(beta, c_old, c) if beta != 0.
+ *   acc = ldexp(v, E_i + F_j - 16);  c = alpha*acc;  c = fma(beta, c_old, c) if beta != 0.
  * Rows/columns containing a non-finite value make the routine return 1.
  */
 /* kc_l of the 0-based level L = l - 2 (L + 1 pairs). */
 ORC_API int orc_ozaki_level_chunk(int L) {
-  int kc = 1024;
-  while (kc * (L + 1) > 1024) kc >>= 1;
+  int kc = 256;
+  while (kc * (L + 1) > 256) kc >>= 1;
   return kc;
 }
 static int row_exponent(const double* v, int64_t n, int64_t stride, int* e) {
@@ -451,9 +456,9 @@ static int row_exponent(const double* v, int64_t n, int64_t stride, int* e) {
 }
 
 static void split_digits(double a, int e, int s, int16_t* d, int64_t dstride) {
-  double r = ldexp(a, -e) * 128.0;
+  double r = ldexp(a, -e) * 256.0;
   for (int t = 0; t < s; ++t) {
-    if (t > 0) r *= 256.0;
+    if (t > 0) r *= 512.0;
     double q = rint(r);
     r -= q;
     d[t * dstride] = (int16_t)q;
@@ -487,11 +492,13 @@ ORC_API int orc_ozaki_split_b(char tb, int64_t k, int64_t n, const double* B, in
   return 0;
 }
 
-/* kcl[L]: chunk length of 0-based level L, each a multiple of the shortest. */
+/* kcl[L]: chunk length of 0-based level L, each a multiple of the shortest.
+ * db: bits per digit (level weight 2^-db), gl: levels per FP64 group,
+ * sh: exponent of the top digit pair (2 * bits of the first digit). */
 static double ozaki_entry(int64_t m, int64_t n, int64_t k, int s, const int64_t* kcl, const int* eA, const int16_t* dA,
-                          const int* eB, const int16_t* dB, int64_t i, int64_t j) {
-  const int ngroups = (s + 3) / 4; /* levels 2..s+1 -> groups of 4 */
-  double G[4] = {0.0, 0.0, 0.0, 0.0};
+                          const int* eB, const int16_t* dB, int64_t i, int64_t j, int db, int gl, int sh) {
+  const int ngroups = (s + gl - 1) / gl; /* levels 2..s+1 -> groups of gl */
+  double G[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   int64_t S[20] = {0};
   int64_t kmin = kcl[0];
   for (int L = 1; L < s; ++L)
@@ -508,18 +515,18 @@ static double ozaki_entry(int64_t m, int64_t n, int64_t k, int s, const int64_t*
       }
     for (int l = 2; l <= s + 1; ++l) {
       if (c1 % kcl[l - 2] != 0 && c1 != k) continue;
-      const int g = (l - 2) / 4, jj = (l - 2) % 4;
-      G[g] = fma((double)S[l], ldexp(1.0, -8 * jj), G[g]);
+      const int g = (l - 2) / gl, jj = (l - 2) % gl;
+      G[g] = fma((double)S[l], ldexp(1.0, -db * jj), G[g]);
       S[l] = 0;
     }
   }
   double hi = 0.0, lo = 0.0;
   for (int g = ngroups - 1; g >= 0; --g) {
-    double w = ldexp(G[g], -32 * g), e;
+    double w = ldexp(G[g], -db * gl * g), e;
     two_sum(hi, w, &hi, &e);
     lo += e;
   }
-  return ldexp(hi + lo, eA[i] + eB[j] - 14);
+  return ldexp(hi + lo, eA[i] + eB[j] - sh);
 }
 
 ORC_API int orc_dgemm_ozaki(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
@@ -547,7 +554,7 @@ ORC_API int orc_dgemm_ozaki(char ta, char tb, int64_t m, int64_t n, int64_t k, d
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
   for (int64_t j = 0; j < n; ++j)
     for (int64_t i = 0; i < m; ++i) {
-      double acc = ozaki_entry(m, n, k, s, kcl, eA, dA, eB, dB, i, j);
+      double acc = ozaki_entry(m, n, k, s, kcl, eA, dA, eB, dB, i, j, 9, s > 6 ? 4 : 3, 16);
       double c = alpha * acc;
       if (beta != 0.0) c = fma(beta, C[i + j * ldc], c);
       C[i + j * ldc] = c;
@@ -568,10 +575,12 @@ done:
  *   X = rint(ldexp(a, -E_i) * 2^(8s-1))   (an integer, |X| < 2^(8s-1));
  *   the s digits are X's signed base-256 digits, least significant first:
  *     d_s = (int8)(X & 255), X = (X - d_s)/256, ..., d_1 = X   (all in [-128, 127]);
- *   so a ~= 2^E_i * sum_t d_t * 2^(-7-8(t-1)), the weights of the BF16 split.
+ *   so a ~= 2^E_i * sum_t d_t * 2^(-7-8(t-1)).
  * Level sums as above, over chunks of kc = 8192 (|S_l| <= s * kc * 2^14 <= 2^30:
- * exact in int32), flushed into the FP64 groups in the same order; the
- * combine and epilogue are those of the BF16 restatement.
+ * exact in int32), flushed in the same order into FP64 groups of four levels,
+ * G_g = fma((double)S_l, 2^(-8j), G_g) with g = (l-2)/4, j = (l-2)%4; the
+ * combine is the BF16 one with w_g = ldexp(G_g, -32g), then
+ * acc = ldexp(v, E_i + F_j - 14).
  */
 #define ORC_I8_CHUNK 8192
 ORC_API int orc_ozaki_i8_chunk(void) { return ORC_I8_CHUNK; }
@@ -651,7 +660,7 @@ ORC_API int orc_dgemm_ozaki_i8(char ta, char tb, int64_t m, int64_t n, int64_t k
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
   for (int64_t j = 0; j < n; ++j)
     for (int64_t i = 0; i < m; ++i) {
-      double acc = ozaki_entry(m, n, k, s, kcl, eA, dA, eB, dB, i, j);
+      double acc = ozaki_entry(m, n, k, s, kcl, eA, dA, eB, dB, i, j, 8, 4, 14);
       double c = alpha * acc;
       if (beta != 0.0) c = fma(beta, C[i + j * ldc], c);
       C[i + j * ldc] = c;

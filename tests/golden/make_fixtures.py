@@ -88,8 +88,8 @@ def main():
                   "source": "oracle", "sha256": digest(D), "samples": samples(D, 256)})
     m, n, k = 48, 40, 70
     A, B, C0 = G.matrix(5, m, k, kind="wide"), G.matrix(6, k, n, kind="wide"), G.matrix(7, m, n)
-    for method, fn in [("ozaki", O.dgemm_ozaki), ("ozaki_int8", O.dgemm_ozaki_i8)]:
-        for s in (3, 7):
+    for method, fn, counts in [("ozaki", O.dgemm_ozaki, (3, 6, 7)), ("ozaki_int8", O.dgemm_ozaki_i8, (3, 7))]:
+        for s in counts:
             R = fn("N", "N", 1.5, A, B, 0.5, C0, s)
             cases.append({"op": method, "nslices": s, "m": m, "n": n, "k": k, "alpha": 1.5, "beta": 0.5,
                           "seeds": [5, 6, 7], "kind": "wide", "source": "oracle", "sha256": digest(R),

@@ -57,9 +57,10 @@ def test_ozaki_bitexact_ops_device(gpu, ta, tb):
 def test_ozaki_exact_for_representable_inputs(gpu):
     rng = np.random.default_rng(9)
     m, n, k = 96, 80, 1500
-    A = np.asfortranarray(rng.integers(-127, 128, (m, k)) * 2.0 ** -7)
-    B = np.asfortranarray(rng.integers(-127, 128, (k, n)) * 2.0 ** -20)
-    A[:, 0], B[0, :] = 127 / 128, 127 * 2.0 ** -20
+    # 9 significant bits at the row/column scale: one BF16 digit (|d| <= 256) each
+    A = np.asfortranarray(rng.integers(-255, 256, (m, k)) * 2.0 ** -8)
+    B = np.asfortranarray(rng.integers(-255, 256, (k, n)) * 2.0 ** -20)
+    A[:, 0], B[0, :] = 255 / 256, 255 * 2.0 ** -20
     C = np.zeros((m, n), order="F")
     jb.gemm_("N", "N", 1.0, A, B, 0.0, C, method="ozaki", nslices=1)
     assert np.array_equal(C, A @ B)
@@ -84,7 +85,7 @@ def test_ozaki_nonfinite_is_an_error(gpu):
 
 
 def test_ozaki_c3_8192_sampled(gpu):
-    """Config: M=N=K=8192, a_ij = u*2^e (e uniform in [-12,12]), s = 3..7 slices.
+    """Config: M=N=K=8192, a_ij = u*2^e (e uniform in [-12,12]), s = 3..8 slices.
     Bit-exact vs the restatement on a 64x64 sample block; error vs Dot2 reported
     per slice count and compared with native FP64 MFMA DGEMM on the same entries."""
     torch = gpu
@@ -108,8 +109,8 @@ def test_ozaki_c3_8192_sampled(gpu):
     nat_err = np.max(np.abs(nat - d2) / absab)
     nat_rel = np.max(np.abs(nat - d2) / np.abs(d2))
     print(f"\nnative: max err/(|A||B|) = {nat_err:.3e}  max rel = {nat_rel:.3e}")
-    prev = None
-    for s in (3, 4, 5, 6, 7):
+    prev, errs = None, {}
+    for s in (3, 4, 5, 6, 7, 8):
         jb.gemm_("N", "N", 1.0, Ad.T, Bd.T, 0.0, Cd.T, method="ozaki", nslices=s)
         torch.cuda.synchronize()
         got = to_host(Cd.T)[np.ix_(ii, jj)]
@@ -121,7 +122,10 @@ def test_ozaki_c3_8192_sampled(gpu):
         if prev is not None:
             assert err < prev
         prev = err
-    assert prev < 64 * nat_err  # s = 7 is within a small factor of native FP64 accuracy
+        errs[s] = (err, rel)
+    # six 9-bit slices (21 products): componentwise at least as accurate as native
+    # FP64 (0.45x on this sample), max relative error within 4x (2.0x)
+    assert errs[6][0] < nat_err and errs[6][1] < 4 * nat_rel
     del Ad, Bd, Cd
     torch.cuda.empty_cache()
 

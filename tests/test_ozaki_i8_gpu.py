@@ -126,7 +126,7 @@ def test_ozaki_i8_nonfinite_is_an_error(gpu):
 def test_ozaki_i8_c3_8192_sampled(gpu):
     """Config: M=N=K=8192, a_ij = u*2^e (e uniform in [-12,12]), s = 5..8 INT8 slices.
     Bit-exact vs the restatement on a 64x64 sample block; error vs Dot2 compared
-    with native FP64 and with the BF16-slice method (s = 7) on the same entries."""
+    with native FP64 and with the BF16-slice method (s = 6) on the same entries."""
     torch = gpu
     n = 8192
     Ad = torch.empty((n, n), dtype=torch.float64, device="cuda")
@@ -149,8 +149,8 @@ def test_ozaki_i8_c3_8192_sampled(gpu):
         return got, float(np.max(np.abs(got - d2) / np.abs(d2)))
 
     _, nat = rel_err("native")
-    _, bf7 = rel_err("ozaki", 7)
-    print(f"\nnative max rel {nat:.3e}; BF16 s=7 {bf7:.3e}")
+    _, bf6 = rel_err("ozaki", 6)
+    print(f"\nnative max rel {nat:.3e}; BF16 s=6 {bf6:.3e}")
     prev = None
     for s in (5, 6, 7, 8):
         got, rel = rel_err("ozaki_int8", s)
