@@ -228,8 +228,32 @@ def cpu_baseline_leg(A_sub_rows, B_sub_cols, A_panel, B_panel, nthreads, grouped
     O.jmul(A_panel, Bp1, nthreads=1)
     t_1 = time.perf_counter() - t0
     flops1 = 2.0 * A_panel.shape[0] * A_panel.shape[1] * Bp1.shape[1]
+    # beside it, the host BLAS (numpy / OpenBLAS dgemm) on a 512-column panel, all threads and one
+    Bq = np.asfortranarray(B_panel[:, :512])
+    blas = {}
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:
+        threadpool_limits = None
+    if threadpool_limits is not None:
+        for th in (nthreads, 1):
+            with threadpool_limits(limits=th, user_api="blas"):
+                A_panel @ Bq
+                ts = []
+                for _ in range(3):
+                    t0 = time.perf_counter()
+                    A_panel @ Bq
+                    ts.append(time.perf_counter() - t0)
+            blas[str(th)] = 2.0 * A_panel.shape[0] * A_panel.shape[1] * Bq.shape[1] / sorted(ts)[1] / 1e12
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        pass
     out["cpu"] = {"mt_tflops": flops / t_mt / 1e12, "mt_seconds": t_mt, "mt_threads": nthreads,
                   "st_tflops": flops1 / t_1 / 1e12, "st_seconds": t_1,
+                  "blas_tflops": blas, "cpu_model": model, "nproc": os.cpu_count(),
                   "panel": f"D[:, 0:{B_panel.shape[1]}] = A*B[:, 0:{B_panel.shape[1]}] of the "
                            f"{A_panel.shape[0]}x{A_panel.shape[1]}x{A_panel.shape[1]} workload "
                            f"(single-thread leg: {Bp1.shape[1]} columns)"}
@@ -534,7 +558,11 @@ def main():
                                 "sample": cpu["panel"] + "; the oracle's C restatement of jmul!'s fma recurrence "
                                           "(register tile per pick_kernel_size, no packing, as the reference)",
                                 "single_thread_tflops": round(cpu["st_tflops"], 5),
-                                "seconds": round(cpu["mt_seconds"] + cpu["st_seconds"], 2)}
+                                "seconds": round(cpu["mt_seconds"] + cpu["st_seconds"], 2),
+                                "host": {"cpu_model": cpu["cpu_model"], "nproc": cpu["nproc"]},
+                                "numpy_openblas_dgemm_tflops": {
+                                    f"{th}_threads": round(v, 4) for th, v in cpu["blas_tflops"].items()},
+                                "numpy_openblas_sample": "A (full) x B[:, 0:512], median of 3 after a warm-up"}
     print(json.dumps(line), flush=True)
 
 

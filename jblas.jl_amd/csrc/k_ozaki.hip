@@ -562,11 +562,17 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
         flush(j, true);  // level j has received its last product of this chunk
     }
   };
-  // nst is a multiple of 2 (the planes' k padding is a multiple of 64); the
-  // guard keeps an odd count correct too (the last chunk then ends in phase 0)
-  for (int st = 0; st < nst; st += 2) {
-    stage(st, std::integral_constant<int, 0>());
-    if (st + 1 < nst) stage(st + 1, std::integral_constant<int, 1>());
+  if constexpr (MODE == 0) {
+    // nst is a multiple of 2 (the planes' k padding is a multiple of 64); the
+    // guard keeps an odd count correct too (the last chunk then ends in phase 0)
+    for (int st = 0; st < nst; st += 2) {
+      stage(st, std::integral_constant<int, 0>());
+      if (st + 1 < nst) stage(st + 1, std::integral_constant<int, 1>());
+    }
+  } else {
+    // no fast levels: one stage per iteration (pairing the stages cost the
+    // 3xBF16 kernel ~20 %)
+    for (int st = 0; st < nst; ++st) stage(st, std::integral_constant<int, 1>());
   }
   if (MODE == 0) {
 #pragma unroll
