@@ -26,7 +26,7 @@
 //      within 2^24 and is exact whatever the MFMA's internal order) level l is
 //      flushed into an FP64 group accumulator right after its last MFMA of the
 //      stage, G_g = fma(S_l, 2^-9j, G_g) with l = 3g+j (exact: G_g is a
-//      multiple of 2^-18 below 2^35 while k*s <= 2^19; for s > 6, l = 4g+j),
+//      multiple of 2^-18 below 2^35 while k*s <= 2^18; for s > 6, l = 4g+j),
 //      so the flush overlaps the remaining MFMAs.  The epilogue combines the groups by
 //      TwoSum from the smallest, rescales by 2^(E_i+F_j-16) and applies
 //      alpha/beta.  These are the oracle restatement's operations in the same

@@ -428,7 +428,7 @@ ORC_API void orc_kernel_acc_d(int64_t M, int64_t N, int64_t P, const double* A, 
  *   At every multiple of the shortest chunk and at k, in increasing l, each
  *   level whose chunk ends there is flushed:  g = (l-2)/3, j = (l-2)%3,
  *     G_g = fma((double)S_l, 2^(-9j), G_g), S_l = 0     (groups of three levels:
- *   G_g is a multiple of 2^-18 and, when k*s <= 2^19, below 2^35, so that every
+ *   G_g is a multiple of 2^-18 and, when k*s <= 2^18, below 2^35, so that every
  *   flush is then exact; for s > 6 the groups hold four levels, g = (l-2)/4,
  *   j = (l-2)%4, and w_g below is ldexp(G_g, -36g)).
  *   v = compensated sum over g from the smallest: w_g = ldexp(G_g, -27g);
