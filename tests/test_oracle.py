@@ -331,6 +331,24 @@ def test_ozaki_exact_when_representable():
         assert np.array_equal(got, A @ B)  # integer-scaled inputs: A@B is exact in float64
 
 
+def ozaki_worst_value(s):
+    """The value whose s 9-bit digits are +256, -256, +256, ... (exact in float64 for s <= 6)."""
+    return sum((256.0 if t % 2 == 0 else -256.0) * 2.0 ** (-8 - 9 * t) for t in range(s))
+
+
+def test_ozaki_worst_case_digits():
+    # every digit at the +-256 bound; every pair of level L has the sign (-1)^L, so the
+    # level sums of a chunk reach (L+1)*kc*2^16 (2^24 for L = 0, 1, 3): still exact
+    s, k = 6, 1024
+    x = ozaki_worst_value(s)
+    e, d = O.ozaki_split_a("N", np.full((3, 5), x), s)
+    assert list(e) == [0, 0, 0] and [int(v) for v in d[:, 0, 0]] == [256, -256, 256, -256, 256, -256]
+    got = O.dgemm_ozaki("N", "N", 1.0, np.full((2, k), x), np.full((k, 3), x), 0.0, np.zeros((2, 3)), s)
+    # exact: k * sum over levels L < s of (L+1) pairs of 2^16 (-1)^L at weight 2^(-16-9L)
+    exact = Fraction(k) * sum(Fraction((L + 1) * (-1) ** L, 2 ** (9 * L)) for L in range(s))
+    assert np.all(np.abs(got - float(exact)) <= np.spacing(float(exact)))
+
+
 def test_ozaki_level_chunks_are_the_longest_exact_ones():
     # level L has L+1 slice pairs of products |d_t d_u| <= 2^16: its FP32 sum over
     # kc k-values is exact while (L+1) * kc * 2^16 <= 2^24; kc is the largest power of two

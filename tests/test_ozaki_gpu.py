@@ -1,6 +1,6 @@
 """GPU parity of the split-BF16 emulated DGEMM and the 3xBF16 SGEMM.
 
-Emulated DGEMM: every FP32 level sum is an exact integer (kc*s <= 1024) and
+Emulated DGEMM: every FP32 level sum is an exact integer (kc*(l+1) <= 256) and
 every FP64 step is the oracle's, so the GPU result must equal the oracle's
 restatement (oracle.dgemm_ozaki) BIT-FOR-BIT, for every slice count, op and
 alpha/beta.  At 8192^3 the check runs on a 64 x 64 block of sampled rows and
@@ -36,6 +36,42 @@ def test_ozaki_bitexact_small(gpu, s):
     ref = O.dgemm_ozaki("N", "N", 1.5, A, B, 0.5, C0, s)
     C = C0.copy(order="F")
     jb.gemm_("N", "N", 1.5, A, B, 0.5, C, method="ozaki", nslices=s)
+    assert np.array_equal(C, ref)
+
+
+def ozaki_worst_value(s):
+    """The value whose s 9-bit digits are +256, -256, +256, ... (exact in float64 for s <= 6)."""
+    return sum((256.0 if t % 2 == 0 else -256.0) * 2.0 ** (-8 - 9 * t) for t in range(s))
+
+
+def test_ozaki_worst_case_digits_bitexact(gpu):
+    """Every digit +-256 and every pair of level L of sign (-1)^L: the FP32 level sums
+    of levels 0, 1 and 3 reach exactly 2^24 at their chunk ends (the exactness limit);
+    rows of A and columns of B carry random signs."""
+    s, m, n, k = 6, 130, 66, 1024
+    x = ozaki_worst_value(s)
+    rng = np.random.default_rng(17)
+    A = np.asfortranarray(rng.choice([-x, x], (m, 1)) * np.ones((1, k)))
+    B = np.asfortranarray(np.ones((k, 1)) * rng.choice([-x, x], (1, n)))
+    ref = O.dgemm_ozaki("N", "N", 1.0, A, B, 0.0, np.zeros((m, n)), s)
+    C = np.zeros((m, n), order="F")
+    jb.gemm_("N", "N", 1.0, A, B, 0.0, C, method="ozaki", nslices=s)
+    assert np.array_equal(C, ref)
+
+
+@pytest.mark.parametrize("s,k", [(6, 2 ** 18), (8, 60000)])
+def test_ozaki_bitexact_long_k(gpu, s, k):
+    """k*s beyond 2^18 with worst-case digits: the FP64 group sums round, so the flush
+    order (per chunk end, increasing level; deferred two-stage levels at s = 6) must
+    be the oracle's for the bits to agree."""
+    m, n = 8, 8
+    x = ozaki_worst_value(min(s, 6))
+    rng = np.random.default_rng(18)
+    A = np.asfortranarray(rng.choice([-x, x], (m, 1)) * np.ones((1, k)))
+    B = np.asfortranarray(np.ones((k, 1)) * rng.choice([-x, x], (1, n)))
+    ref = O.dgemm_ozaki("N", "N", 1.0, A, B, 0.0, np.zeros((m, n)), s)
+    C = np.zeros((m, n), order="F")
+    jb.gemm_("N", "N", 1.0, A, B, 0.0, C, method="ozaki", nslices=s)
     assert np.array_equal(C, ref)
 
 
