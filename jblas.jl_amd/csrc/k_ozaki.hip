@@ -19,9 +19,9 @@
 //      set per level l = t+u (pairs t+u <= s+1) on v_mfma_f32_32x32x16_bf16,
 //      pairs in square order (step j: the new slices A_j, B_j against all
 //      earlier ones, so a step needs two new fragments).  Operand stages stream
-//      into a 3-slot LDS ring by direct global->LDS loads
-//      (global_load_lds_dwordx4), two stages ahead, one piece per step, one
-//      s_barrier per stage.  Every kc_l k-values (kc_l*(l+1) <= 256 for the
+//      into an LDS ring of 4 slots (s <= 6; 3 above, whose stages 4 would not
+//      fit) by direct global->LDS loads (global_load_lds_dwordx4), three (two)
+//      stages ahead, one piece per step, one s_barrier per stage.  Every kc_l k-values (kc_l*(l+1) <= 256 for the
 //      l+1 pairs of level l, products |d_t d_u| <= 2^16, so a level sum stays
 //      within 2^24 and is exact whatever the MFMA's internal order) level l is
 //      flushed into an FP64 group accumulator right after its last MFMA of the
@@ -56,7 +56,7 @@ typedef int i16v __attribute__((ext_vector_type(16)));
 
 constexpr int QBM = 128, QBN = 64;  // block tile (4/WB x 2 waves of 32*WB x 32)
 constexpr int QBK = 16;             // k per LDS stage = one MFMA k-step
-constexpr int QSLOTS = 3;           // LDS ring depth (prefetch distance 2)
+constexpr int QSLOTS = 3;           // LDS ring depth (prefetch distance 2; 4 and 3 in MODE 0 with s <= 6)
 constexpr int QPAD_K = 64;          // k padding of the planes (split tile width)
 #ifndef OZ_WB
 #define OZ_WB 1  // emulated FP64: 8 waves of 32 x 32 (the FP64 group accumulators need the registers)
@@ -406,7 +406,12 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   // in MODE 2, whose stages hold 32 k.  All powers of two.
   auto kc_stages = [](int L) { return (MODE == 2) ? 8192 / 32 : level_chunk(L) / QBK; };
   typedef typename std::conditional<MODE == 2, i16v, f16v>::type AccV;
-  __shared__ __attribute__((aligned(1024))) char ring[QSLOTS * STAGE_B];
+  // LDS ring depth: four slots (three stages of loads in flight) where they fit
+  // the 160 KB of LDS in the emulated-FP64 mode (s <= 6: 2.5-3 % faster at
+  // 8192^3 than three), else three
+  constexpr int NSLOT = (MODE == 0 && 4 * STAGE_B <= 160 * 1024) ? 4 : QSLOTS;
+  constexpr int PD = NSLOT - 1;  // prefetch distance
+  __shared__ __attribute__((aligned(1024))) char ring[NSLOT * STAGE_B];
 
   const int nblocks = p.tiles_m * p.tiles_n;
   const int tile = xcd_remap(blockIdx.x, nblocks, 8);
@@ -497,23 +502,41 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   auto fast = [&](const int L) { return MODE == 0 && S <= 6 && kc_stages(L) == 2; };  // (S > 6 would spill)
 
   const int nst = (int)(p.Kpad / QBK);
-  issue(0, 0);
-  if (nst > 1) issue(1, 1);
+  if constexpr (PD > 2) {
+#pragma unroll
+    for (int i = 0; i < PD; ++i)
+      if (i < nst) issue(i, i);
+  } else {
+    issue(0, 0);
+    if (nst > 1) issue(1, 1);
+  }
   auto stage = [&](const int st, auto phase) {
     constexpr int PH = decltype(phase)::value;
-    // this stage landed; the next stage's loads (this wave's count) may be in flight
-    if (st + 1 < nst) {
-      if (wave < NWL % NWAVES) wait_vmcnt<WL_MAX>();
-      else wait_vmcnt<WL_MIN>();
+    // this stage landed; the later stages' loads (this wave's count each) may be in flight
+    if constexpr (PD > 2) {
+      const int ahead = min(PD - 1, nst - 1 - st);  // stages issued beyond st, still in flight
+      if (ahead >= 2) {
+        if (wave < NWL % NWAVES) wait_vmcnt<2 * WL_MAX>();
+        else wait_vmcnt<2 * WL_MIN>();
+      } else if (ahead == 1) {
+        if (wave < NWL % NWAVES) wait_vmcnt<WL_MAX>();
+        else wait_vmcnt<WL_MIN>();
+      }
+      else wait_vmcnt<0>();
+    } else {
+      if (st + 1 < nst) {
+        if (wave < NWL % NWAVES) wait_vmcnt<WL_MAX>();
+        else wait_vmcnt<WL_MIN>();
+      }
+      else wait_vmcnt<0>();
     }
-    else wait_vmcnt<0>();
     block_barrier();  // every wave's loads of stage st landed, every wave is done with st-1
-    // Stage st+2 goes into the slot of st-1, one piece per step below: a burst of
+    // Stage st+PD goes into the slot of st-1, one piece per step below: a burst of
     // loads right after the barrier fills the memory pipeline's queue and holds
     // up the wave's MFMA issue behind it.
-    const bool prefetch = st + 2 < nst;
-    const int pslot = (st + 2) % QSLOTS;
-    const u4* L = reinterpret_cast<const u4*>(ring + (st % QSLOTS) * STAGE_B);
+    const bool prefetch = st + PD < nst;
+    const int pslot = (st + PD) % NSLOT;
+    const u4* L = reinterpret_cast<const u4*>(ring + (st % NSLOT) * STAGE_B);
     const int st1 = st + 1;  // a level's chunk ends after this stage if st1 is a multiple of its length
     const bool last_stage = st1 == nst;
     // Pairs run in square order: step j multiplies the new slices A_j and B_j
@@ -530,7 +553,7 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
     for (int j = 0; j < S; ++j) {
       if (prefetch) {
 #pragma unroll
-        for (int i = PPS * j; i < PPS * (j + 1) && i < WL_MAX; ++i) issue_piece(st + 2, pslot, i);
+        for (int i = PPS * j; i < PPS * (j + 1) && i < WL_MAX; ++i) issue_piece(st + PD, pslot, i);
       }
       if (j + 1 < S) {
 #pragma unroll
