@@ -432,8 +432,12 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   // 1 KB piece (row lane/2, physical half lane%2) and so fetches the logical
   // half (lane%2) ^ bit 3 of its row from global memory.
   const int dma_off = (lane / 2) * QBK + 8 * ((lane % 2) ^ ((lane >> 4) & 1));
-  const uint16_t* gA = p.pa + (int64_t)tm * QBM * p.Kpad + dma_off;
-  const uint16_t* gB = p.pb + (int64_t)tn * QBN * p.Kpad + dma_off;
+  // Wave-uniform bases: a piece's address is the uniform part (formed in scalar
+  // registers) plus the lane's offset, added last.  Adding the lane offset
+  // first put a 64-bit vector add (and selects) in front of every load; this
+  // form took the s = 6 kernel from 19.9 to 19.0 ms at 8192^3.
+  const uint16_t* gA = p.pa + (int64_t)tm * QBM * p.Kpad;
+  const uint16_t* gB = p.pb + (int64_t)tn * QBN * p.Kpad;
   // Piece i of this wave's share of stage ks (pieces j = wave + NWAVES*i; A:
   // plane t, 4 pieces of 1 KB; B: plane t, 2 pieces) -> LDS slot `slot`.
   auto issue_piece = [&](int ks, int slot, int i) {
@@ -450,7 +454,8 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
       src = gB + t * p.strideB + (int64_t)ks * (QBN * QBK) + q * 512;
       loff = t * PLANE_B + QBM * QBK * 2 + q * 1024;
     }
-    __builtin_amdgcn_global_load_lds((const void*)src, (JB_LDS void*)(ring + slot * STAGE_B + loff), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void*)(src + (uint32_t)dma_off), (JB_LDS void*)(ring + slot * STAGE_B + loff),
+                                     16, 0, 0);
   };
   auto issue = [&](int ks, int slot) {
 #pragma unroll

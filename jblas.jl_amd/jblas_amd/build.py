@@ -20,7 +20,10 @@ LIB = os.path.join(LIBDIR, "libjblas_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # Packed FP32 instructions are disabled for every HIP translation unit.
+# -Werror=shorten-64-to-32: an implicit 64 -> 32-bit conversion (e.g. a pointer through an int-typed
+# builtin such as __builtin_amdgcn_readfirstlane) is a build error, not a wrong device address
 CFLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+          "-Werror=shorten-64-to-32",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
