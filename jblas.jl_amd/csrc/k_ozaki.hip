@@ -330,11 +330,21 @@ __global__ __launch_bounds__(256) void split_f32_kernel(const float* __restrict_
   const int t = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
   const bool rows_contig = (sr == 1);
-#pragma unroll 4
-  for (int j = 0; j < 16; ++j) {
-    const int r = rows_contig ? (t % 64) : (t / 64 + 4 * j);
-    const int k = rows_contig ? (t / 64 + 4 * j) : (t % 64);
-    tile[r][k] = (r0 + r < R && k0 + k < K) ? P[(r0 + r) * sr + (k0 + k) * sk] : 0.0f;
+  {
+    // all 16 loads of the thread in flight before the LDS stores
+    float v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int r = rows_contig ? (t % 64) : (t / 64 + 4 * j);
+      const int k = rows_contig ? (t / 64 + 4 * j) : (t % 64);
+      v[j] = (r0 + r < R && k0 + k < K) ? P[(r0 + r) * sr + (k0 + k) * sk] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int r = rows_contig ? (t % 64) : (t / 64 + 4 * j);
+      const int k = rows_contig ? (t / 64 + 4 * j) : (t % 64);
+      tile[r][k] = v[j];
+    }
   }
   __syncthreads();
 #pragma unroll
