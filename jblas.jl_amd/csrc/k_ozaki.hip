@@ -517,7 +517,8 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   void* eC = p.C;
   asm volatile("" : "+s"(eN), "+s"(eldc), "+s"(ealpha), "+s"(ebeta), "+s"(eC));
 
-  // Flush of level j: G_g += S_j * 2^(-DB*(j%GL)), exact (S_j is an integer below 2^24).
+  // Flush of level j: G_g += S_j * 2^(-DB*(j%GL)) (S_j an exact integer, |S_j| <= 2^24);
+  // the update itself is exact while G_g fits 53 bits (groups of three, k*s <= 2^18).
   auto flush = [&](const int j, const bool zero) {
     const double wj = __builtin_ldexp(1.0, -DB * (j % GL));  // folds to a constant
 #pragma unroll
