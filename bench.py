@@ -245,6 +245,31 @@ def cpu_baseline_leg(A_sub_rows, B_sub_cols, A_panel, B_panel, nthreads, grouped
                     A_panel @ Bq
                     ts.append(time.perf_counter() - t0)
             blas[str(th)] = 2.0 * A_panel.shape[0] * A_panel.shape[1] * Bq.shape[1] / sorted(ts)[1] / 1e12
+    # the reference's CPU protocol: median of repetitions at 256^3, 1024^3 (5 each) and
+    # 4096^3 (3; the one-thread restatement there is skipped: ~20 s per repetition)
+    sizes = {}
+    rng = np.random.default_rng(7)
+    for n, reps in ((256, 5), (1024, 5), (4096, 3)):
+        A = np.asfortranarray(rng.uniform(-1.0, 1.0, (n, n)))
+        B = np.asfortranarray(rng.uniform(-1.0, 1.0, (n, n)))
+        row = {}
+
+        def med(fn):
+            fn()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t0)
+            return round(2.0 * n ** 3 / sorted(ts)[len(ts) // 2] / 1e9, 3)
+        row[f"jmul_restatement_{nthreads}t_gflops"] = med(lambda: O.jmul(A, B, nthreads=nthreads))
+        if n <= 1024:
+            row["jmul_restatement_1t_gflops"] = med(lambda: O.jmul(A, B, nthreads=1))
+        if threadpool_limits is not None:
+            for th in (nthreads, 1):
+                with threadpool_limits(limits=th, user_api="blas"):
+                    row[f"blas_{th}t_gflops"] = med(lambda: A @ B)
+        sizes[f"{n}^3"] = row
     model = None
     try:
         with open("/proc/cpuinfo") as f:
@@ -253,7 +278,7 @@ def cpu_baseline_leg(A_sub_rows, B_sub_cols, A_panel, B_panel, nthreads, grouped
         pass
     out["cpu"] = {"mt_tflops": flops / t_mt / 1e12, "mt_seconds": t_mt, "mt_threads": nthreads,
                   "st_tflops": flops1 / t_1 / 1e12, "st_seconds": t_1,
-                  "blas_tflops": blas, "cpu_model": model, "nproc": os.cpu_count(),
+                  "blas_tflops": blas, "cpu_model": model, "nproc": os.cpu_count(), "sizes": sizes,
                   "panel": f"D[:, 0:{B_panel.shape[1]}] = A*B[:, 0:{B_panel.shape[1]}] of the "
                            f"{A_panel.shape[0]}x{A_panel.shape[1]}x{A_panel.shape[1]} workload "
                            f"(single-thread leg: {Bp1.shape[1]} columns)"}
@@ -562,7 +587,10 @@ def main():
                                 "host": {"cpu_model": cpu["cpu_model"], "nproc": cpu["nproc"]},
                                 "numpy_openblas_dgemm_tflops": {
                                     f"{th}_threads": round(v, 4) for th, v in cpu["blas_tflops"].items()},
-                                "numpy_openblas_sample": "A (full) x B[:, 0:512], median of 3 after a warm-up"}
+                                "numpy_openblas_sample": "A (full) x B[:, 0:512], median of 3 after a warm-up",
+                                "sizes_gflops": cpu["sizes"],
+                                "sizes_protocol": "uniform [-1,1) square operands; median of 5 (256^3, 1024^3) or 3 "
+                                                  "(4096^3) after a warm-up; restatement = the oracle's jmul! port"}
     print(json.dumps(line), flush=True)
 
 
