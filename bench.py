@@ -319,8 +319,8 @@ def cpu_baseline_leg(A_sub_rows, B_sub_cols, A_panel, B_panel, nthreads, grouped
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
-                                   "DESIGN.md and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the benchmark runs on one GPU only: sharding the GEMM across GPUs "
+                                   "(by tiles of C) is not launched", "n_gpus": args.gpus}))
         sys.exit(2)
     import numpy as np
     import torch
