@@ -207,81 +207,34 @@ def profiled_traffic(key):
         return None
 
 
-def cpu_baseline_leg(A_sub_rows, B_sub_cols, A_panel, B_panel, nthreads, grouped_samples=None, lines=None):
+def cpu_baseline_leg(A_sub_rows, B_sub_cols, n, panel_cols, nthreads, grouped_samples=None, lines=None):
     """CPU side, rank 0 only: (1) the oracle's restatement of the reference
-    jmul! recurrence timed on a bounded panel of the same workload, (2) the
+    jmul! recurrence timed on a bounded panel of the same workload, beside the
+    host BLAS, in a child process (oracle/cpu_timing.py: no GPU runtime threads,
+    OpenMP threads pinned from the start, warmed legs, medians of 5), (2) the
     Dot2 FP64 oracle on the sampled rows x columns, on whole rows and columns
     of C (`lines` = (A row block, B, A stored by rows, B column block)) and on
     sampled problems of the batched launch, used only as the checker."""
+    import subprocess
+
     import numpy as np
 
     from oracle import oracle as O
 
     out = {}
-    # (1) reference CPU path (jmul! restatement), all host threads and one thread
-    t0 = time.perf_counter()
-    O.jmul(A_panel, B_panel, nthreads=nthreads)
-    t_mt = time.perf_counter() - t0
-    flops = 2.0 * A_panel.shape[0] * A_panel.shape[1] * B_panel.shape[1]
-    Bp1 = np.asfortranarray(B_panel[:, : int(os.environ.get("JB_CPU_PANEL_1T", "320"))])
-    t0 = time.perf_counter()
-    O.jmul(A_panel, Bp1, nthreads=1)
-    t_1 = time.perf_counter() - t0
-    flops1 = 2.0 * A_panel.shape[0] * A_panel.shape[1] * Bp1.shape[1]
-    # beside it, the host BLAS (numpy / OpenBLAS dgemm) on a 512-column panel, all threads and one
-    Bq = np.asfortranarray(B_panel[:, :512])
-    blas = {}
-    try:
-        from threadpoolctl import threadpool_limits
-    except ImportError:
-        threadpool_limits = None
-    if threadpool_limits is not None:
-        for th in (nthreads, 1):
-            with threadpool_limits(limits=th, user_api="blas"):
-                A_panel @ Bq
-                ts = []
-                for _ in range(3):
-                    t0 = time.perf_counter()
-                    A_panel @ Bq
-                    ts.append(time.perf_counter() - t0)
-            blas[str(th)] = 2.0 * A_panel.shape[0] * A_panel.shape[1] * Bq.shape[1] / sorted(ts)[1] / 1e12
-    # the reference's CPU protocol: median of repetitions at 256^3, 1024^3 (5 each) and
-    # 4096^3 (3; the one-thread restatement there is skipped: ~20 s per repetition)
-    sizes = {}
-    rng = np.random.default_rng(7)
-    for n, reps in ((256, 5), (1024, 5), (4096, 3)):
-        A = np.asfortranarray(rng.uniform(-1.0, 1.0, (n, n)))
-        B = np.asfortranarray(rng.uniform(-1.0, 1.0, (n, n)))
-        row = {}
-
-        def med(fn):
-            fn()
-            ts = []
-            for _ in range(reps):
-                t0 = time.perf_counter()
-                fn()
-                ts.append(time.perf_counter() - t0)
-            return round(2.0 * n ** 3 / sorted(ts)[len(ts) // 2] / 1e9, 3)
-        row[f"jmul_restatement_{nthreads}t_gflops"] = med(lambda: O.jmul(A, B, nthreads=nthreads))
-        if n <= 1024:
-            row["jmul_restatement_1t_gflops"] = med(lambda: O.jmul(A, B, nthreads=1))
-        if threadpool_limits is not None:
-            for th in (nthreads, 1):
-                with threadpool_limits(limits=th, user_api="blas"):
-                    row[f"blas_{th}t_gflops"] = med(lambda: A @ B)
-        sizes[f"{n}^3"] = row
-    model = None
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
-    except OSError:
-        pass
-    out["cpu"] = {"mt_tflops": flops / t_mt / 1e12, "mt_seconds": t_mt, "mt_threads": nthreads,
-                  "st_tflops": flops1 / t_1 / 1e12, "st_seconds": t_1,
-                  "blas_tflops": blas, "cpu_model": model, "nproc": os.cpu_count(), "sizes": sizes,
-                  "panel": f"D[:, 0:{B_panel.shape[1]}] = A*B[:, 0:{B_panel.shape[1]}] of the "
-                           f"{A_panel.shape[0]}x{A_panel.shape[1]}x{A_panel.shape[1]} workload "
-                           f"(single-thread leg: {Bp1.shape[1]} columns)"}
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores", OMP_WAIT_POLICY="passive",
+               OMP_NUM_THREADS=str(nthreads))
+    r = subprocess.run([sys.executable, "-m", "oracle.cpu_timing", "--n", str(n), "--panel", str(panel_cols),
+                        "--panel1t", os.environ.get("JB_CPU_PANEL_1T", "320"), "--threads", str(nthreads)],
+                       cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU timing child failed ({r.returncode}): {r.stderr[-2000:]}")
+    t = json.loads(r.stdout.strip().splitlines()[-1])
+    out["cpu"] = {"mt_tflops": t["panel"]["mt_tflops"], "mt_seconds": t["panel"]["mt_seconds"],
+                  "mt_threads": t["threads_used"], "st_tflops": t["panel"]["st_tflops"],
+                  "st_seconds": t["panel"]["st_seconds"], "blas_tflops": t["blas_tflops"],
+                  "cpu_model": t["cpu_model"], "nproc": t["nproc"], "sizes": t["sizes"], "omp": t["omp"],
+                  "panel": t["panel"]["sample"]}
     # (2) checker: Dot2 on the sampled block
     out["ref"] = O.gemm_dot2("N", "N", 1.0, A_sub_rows, B_sub_cols, 0.0,
                              np.zeros((A_sub_rows.shape[0], B_sub_cols.shape[1])), nthreads=nthreads)
@@ -419,6 +372,38 @@ def main():
                               "out": C.clone() if not args.skip_cpu else None})
                 log(f"  table {meth} s={ss}: {ms_s:.3f} ms  {flops / ms_s / 1e9:.2f} TF")
 
+    # ---- Float32 through the 3xBF16 split at the same size (after the timed region) ----
+    f32 = None
+    if args.only is None and not args.skip_table:
+        f32 = {"rows": []}
+        A32, B32 = A.float(), B.float()  # the wide-range FP64 streams rounded to float (jb_fill_wide_f32_dev's values)
+        C32 = torch.zeros((n, n), dtype=torch.float32, device="cuda")
+        for meth, nprod in (("bf16x3", 3), ("bf16x3", 6), ("bf16x3", 9), ("native", None)):
+            def f(meth=meth, nprod=nprod):
+                jb.gemm_("N", "N", 1.0, A32.T, B32.T, 0.0, C32.T, method=meth, nprod=nprod or 6)
+            for _ in range(args.warmup):
+                f()
+            _lib.kernel_timing_reset()
+            _lib.kernel_timing(True)
+            ms_f = timed(torch, f, max(2, args.steps // 2))
+            _lib.kernel_timing(False)
+            kname = "bf16x3_gemm" if meth == "bf16x3" else "sgemm_mfma"
+            k_ms, k_n = _lib.kernel_time(kname)
+            row = {"method": meth, "nprod": nprod, "ms": round(ms_f, 3), "tflops": round(flops / ms_f / 1e9, 2),
+                   "kernel_ms": round(k_ms / max(k_n, 1), 4), "out": C32.clone() if not args.skip_cpu else None}
+            if meth == "bf16x3":
+                hw = nprod * flops / (k_ms / max(k_n, 1) * 1e-3) / 1e12
+                row["roofline"] = {"kernel": f"slice_pair_gemm_kernel<1,3,{ {3: 3, 6: 4, 9: 6}[nprod] },2>",
+                                   "bound": "mfma", "achieved": round(hw, 2),
+                                   "peak": round(info["peak_bf16_tflops"], 1), "unit": "TFLOP/s",
+                                   "frac": round(hw / info["peak_bf16_tflops"], 4)}
+            f32["rows"].append(row)
+            log(f"  f32 {meth} nprod={nprod}: {ms_f:.3f} ms  {flops / ms_f / 1e9:.2f} TF")
+        if args.skip_cpu:
+            for row in f32["rows"]:
+                row.pop("out", None)
+        del C32
+
     # ---- batched and rrule workloads (after the timed region) ----
     extra, gsamples = None, None
     if args.only is None and not args.skip_grouped:
@@ -447,8 +432,8 @@ def main():
         # 16 whole rows and 16 whole columns of C, besides the 64 x 64 sampled block
         li, lj = ii[::4], jj[::4]
         A_by_rows = np.ascontiguousarray(Ah)
-        leg = cpu_baseline_leg(A_rows, B_cols, np.asfortranarray(Ah), np.asfortranarray(Bh[:, :panel_cols]), nthreads,
-                               gsamples, lines=(A_rows[::4], Bh, A_by_rows, B_cols[:, ::4]))
+        leg = cpu_baseline_leg(A_rows, B_cols, n, panel_cols, nthreads, gsamples,
+                               lines=(A_rows[::4], Bh, A_by_rows, B_cols[:, ::4]))
         absab_rows = np.abs(A_rows[::4]) @ np.abs(Bh)
         absab_cols = np.abs(Ah) @ np.abs(B_cols[:, ::4])
         del Ah, Bh, A_by_rows
@@ -463,11 +448,15 @@ def main():
             |C*| > 1e-3 max|C*|, count of entries excluded from that) on the Dot2-checked sample."""
             Cm = out_t.cpu().numpy().T
             got = np.concatenate([Cm[np.ix_(ii, jj)].ravel(), Cm[li, :].ravel(), Cm[:, lj].ravel()])
-            d = np.abs(got - ref_all)
-            big = np.abs(ref_all) > 1e-3 * np.max(np.abs(ref_all))
-            return (float(np.max(d / np.abs(ref_all))), float(np.max(d / absab_all)),
-                    float(np.linalg.norm(got - ref_all) / np.linalg.norm(ref_all)),
-                    float(np.max(d[big] / np.abs(ref_all[big]))), int(np.count_nonzero(~big)))
+            return err_of(got, ref_all, absab_all)
+
+        def err_of(got, ref_v, absab_v):
+            d = np.abs(got - ref_v)
+            big = np.abs(ref_v) > 1e-3 * np.max(np.abs(ref_v))
+            return (float(np.max(d / np.abs(ref_v))), float(np.max(d / absab_v)),
+                    float(np.linalg.norm(got - ref_v) / np.linalg.norm(ref_v)),
+                    float(np.max(d[big] / np.abs(ref_v[big]))), int(np.count_nonzero(~big)),
+                    float(np.mean(d[big] / np.abs(ref_v[big]))))
 
         if "ozaki" in result:
             errs["ozaki"] = err(C_oz)
@@ -476,9 +465,34 @@ def main():
         if "int8" in result:
             errs["int8"] = err(C_i8)
         for row in table:
-            row["max_rel_err"], row["max_err_over_absAB"], row["normwise_err"] = err(row.pop("out"))[:3]
+            e = err(row.pop("out"))
+            row["max_rel_err"], row["max_err_over_absAB"], row["normwise_err"] = e[:3]
+            row["max_rel_err_excluding_small"], row["mean_rel_err_excluding_small"] = e[3], e[5]
             if "native" in errs:
                 row["err_ratio_vs_native"] = round(row["max_rel_err"] / errs["native"][0], 3)
+        if f32 is not None:
+            # FP64 Dot2 oracle on the float inputs (exact in FP64), same sample
+            from oracle import oracle as O
+            A32h = A32.double().cpu().numpy().T
+            B32h = B32.double().cpu().numpy().T
+            r_blk = O.gemm_dot2("N", "N", 1.0, np.asfortranarray(A32h[ii, :]), np.asfortranarray(B32h[:, jj]), 0.0,
+                                np.zeros((ii.size, jj.size)), nthreads=nthreads)
+            r_rows = O.gemm_dot2("T", "N", 1.0, np.asfortranarray(A32h[li, :].T), B32h, 0.0,
+                                 np.zeros((li.size, n)), nthreads=nthreads)
+            r_cols = O.gemm_dot2("T", "N", 1.0, np.asfortranarray(B32h[:, lj]), np.ascontiguousarray(A32h).T, 0.0,
+                                 np.zeros((lj.size, n)), nthreads=nthreads)
+            ref32 = np.concatenate([r_blk.ravel(), r_rows.ravel(), r_cols.T.ravel()])
+            aA, aB = np.abs(A32h), np.abs(B32h)
+            absab32 = np.concatenate([(aA[ii, :] @ aB[:, jj]).ravel(), (aA[li, :] @ aB).ravel(),
+                                      (aA @ aB[:, lj]).ravel()])
+            del A32h, B32h, aA, aB
+            for row in f32["rows"]:
+                Cm = row.pop("out").cpu().numpy().T.astype(np.float64)
+                got = np.concatenate([Cm[np.ix_(ii, jj)].ravel(), Cm[li, :].ravel(), Cm[:, lj].ravel()])
+                e = err_of(got, ref32, absab32)
+                row.update({"max_err_over_absAB": e[1], "max_rel_err": e[0], "normwise_err": e[2],
+                            "max_rel_err_excluding_small": e[3], "mean_rel_err_excluding_small": e[5],
+                            "bar_2K_u32": 2 * n * 2.0 ** -24})
         cpu = leg["cpu"]
         if extra is not None and "grouped_errors" in leg:
             for method, e in leg["grouped_errors"].items():
@@ -547,7 +561,7 @@ def main():
                                 f"rows and 16 whole columns ({4096 + 32 * n} entries)")
         line["error_metrics"] = {
             name: {"componentwise_scaled": e[1], "max_rel": e[0], "normwise": e[2],
-                   "max_rel_excluding_small": e[3], "excluded_entries": e[4]}
+                   "max_rel_excluding_small": e[3], "excluded_entries": e[4], "mean_rel_excluding_small": e[5]}
             for name, e in (("emulated", errs.get("ozaki")), ("native", errs.get("native")),
                             ("emulated_int8", errs.get("int8"))) if e is not None}
     # roofline of the dominant kernel
@@ -573,6 +587,10 @@ def main():
                                    "frac": round(flops / (nat["kernel_ms"] * 1e-3) / 1e12 / info["peak_fp64_tflops"], 4)}
     if table:
         line["nslices_table"] = table
+    if f32:
+        line["sgemm_f32_8192"] = {"workload": f"sgemm {n}x{n}x{n} NN, Float32 inputs = the wide-range operands "
+                                              f"rounded to float; 3xBF16 split with 3, 6, 9 slice products on BF16 "
+                                              f"MFMA, and native FP32 MFMA", "rows": f32["rows"]}
     if extra:
         line["batched_and_rrule"] = extra
     if secondary:
@@ -587,10 +605,12 @@ def main():
                                 "host": {"cpu_model": cpu["cpu_model"], "nproc": cpu["nproc"]},
                                 "numpy_openblas_dgemm_tflops": {
                                     f"{th}_threads": round(v, 4) for th, v in cpu["blas_tflops"].items()},
-                                "numpy_openblas_sample": "A (full) x B[:, 0:512], median of 3 after a warm-up",
+                                "numpy_openblas_sample": "A (full) x B[:, 0:512], median of 5 after a warm-up",
                                 "sizes_gflops": cpu["sizes"],
-                                "sizes_protocol": "uniform [-1,1) square operands; median of 5 (256^3, 1024^3) or 3 "
-                                                  "(4096^3) after a warm-up; restatement = the oracle's jmul! port"}
+                                "sizes_protocol": "uniform [-1,1) square operands; median of 5 after a warm-up; "
+                                                  "restatement = the oracle's jmul! port; run in a child process "
+                                                  "without GPU runtime threads, OpenMP threads pinned "
+                                                  f"({cpu['omp']})"}
     print(json.dumps(line), flush=True)
 
 

@@ -28,7 +28,8 @@
  *   - Functions without the _dev suffix take HOST pointers and are synchronous:
  *     they stage operands through the library's device workspace (the drop-in
  *     for jmul!/gemm! on Julia Arrays).  *_dev functions take DEVICE pointers
- *     and are asynchronous on `stream` (a hipStream_t; NULL = default stream).
+ *     and are asynchronous on `stream` (a hipStream_t; NULL = default stream),
+ *     except the one-shot grouped forms named below, which synchronise.
  *   - Calls are serialised inside the library (one mutex); the device workspace
  *     is owned by the library and released by jb_finalize().
  */
@@ -168,7 +169,15 @@ int jb_dgemm_recurrence_dev(char transA, char transB, int64_t m, int64_t n, int6
  * pairs (t+u <= nslices+1) multiplied on BF16 MFMA with exact FP32 level sums,
  * level sums combined in compensated FP64.  Six slices (21 pair products) are
  * as accurate as native FP64 componentwise on wide-range 8192^3 inputs.  The 15th
- * argument (nslices) is validated as argument 14 in the _dev form too. */
+ * argument (nslices) is validated as argument 14 in the _dev form too.
+ * Non-finite operands: every element of C fed by a row of op(A) or a column of
+ * op(B) that holds Inf or NaN is set to NaN (the other elements are computed
+ * as usual).  The host-pointer form then returns JB_ERR_NONFINITE; the _dev
+ * form does not synchronise and returns JB_OK, and the condition is reported by
+ * jb_nonfinite_check().  The _dev form can be captured into a hipGraph once a
+ * call of the same or a larger size has run outside the capture (the
+ * workspace does not grow during capture); replays must not overlap other
+ * calls of the library's emulated methods. */
 int jb_dgemm_ozaki(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
                    const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                    double* C, int64_t ldc, int nslices);
@@ -198,13 +207,25 @@ int jb_sgemm_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, floa
                  int64_t ldc, void* stream);
 
 /* FP32 emulated with 3 BF16 slices per element (hi, mid, lo) on BF16 MFMA and
- * FP32 accumulation; nprod in {3, 6, 9} slice products (6 = all pairs t+u <= 4). */
+ * FP32 accumulation; nprod in {3, 6, 9} slice products (6 = all pairs t+u <= 4).
+ * Finite inputs up to FLT_MAX are split exactly.  An Inf or NaN operand element
+ * becomes NaN in every product it enters (where native FP32 could give +-Inf)
+ * and is reported as for the emulated FP64 methods (JB_ERR_NONFINITE from the
+ * host-pointer form, jb_nonfinite_check() after the _dev form). */
 int jb_sgemm_bf16x3(char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha,
                     const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
                     float* C, int64_t ldc, int nprod);
 int jb_sgemm_bf16x3_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, float alpha,
                         const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
                         float* C, int64_t ldc, int nprod, void* stream);
+
+/* Reports Inf/NaN operands met by emulated _dev calls (split-BF16, split-INT8,
+ * 3xBF16) since the previous check, on any stream: synchronises `stream`, then
+ * returns JB_ERR_NONFINITE (and clears the condition) if any such call met one
+ * (the host-pointer forms, which report their own, raise it too),
+ * else JB_OK.  Not allowed while `stream` is being captured (JB_ERR_STATE).  No
+ * counterpart in the reference (its jmul! has no error reporting). */
+int jb_nonfinite_check(void* stream);
 
 /* ---- batched / grouped GEMM ---------------------------------------------------- */
 /* One problem of a grouped launch.  A, B, C are DEVICE pointers of the method's
@@ -225,11 +246,15 @@ typedef struct jb_gemm_desc {
 
 /* All `count` problems in one persistent launch (tile shape and split-K per
  * problem from the autotuning table).  `descs` is a HOST array.
- * Methods: JB_F64_NATIVE, JB_F32_BF16X3, JB_F32_NATIVE. */
+ * Methods: JB_F64_NATIVE, JB_F32_BF16X3, JB_F32_NATIVE.  This one-shot form
+ * uploads the descriptors and synchronises `stream` before it returns (use a
+ * plan for asynchronous, graph-replayed launches). */
 int jb_gemm_grouped_dev(int method, int64_t count, const jb_gemm_desc* descs, void* stream);
 
 /* A grouped plan: descriptors uploaded once, the persistent launch captured in
- * a hipGraph; jb_grouped_plan_launch replays it on `stream`. */
+ * a hipGraph; jb_grouped_plan_launch replays it on `stream`.  Replays of one
+ * plan share its work counters, so a replay waits (on the device, through an
+ * event) for the previous replay of the same plan, whatever its stream. */
 typedef struct jb_grouped_plan jb_grouped_plan;
 int jb_grouped_plan_create(int method, int64_t count, const jb_gemm_desc* descs, void* stream,
                            jb_grouped_plan** plan);
@@ -242,7 +267,9 @@ int jb_grouped_plan_destroy(jb_grouped_plan* plan);
  * visited once in decreasing order of their flops (those under 0.5 % of the
  * total keep the static rule), and a candidate is kept only if it makes the
  * group >= 1 % faster.  Improved buckets go into the in-memory table (replaces
- * the static pick_kernel_size / blocking_structure choice). */
+ * the static pick_kernel_size / blocking_structure choice).  The trials write
+ * to scratch copies of the outputs: the group's C matrices are neither read
+ * nor written.  Synchronous. */
 int jb_autotune_grouped(int method, int64_t count, const jb_gemm_desc* descs, int reps,
                         void* stream);
 /* Persist / restore the table as JSON (keyed by device, arch, method, bucket). */
@@ -260,8 +287,9 @@ int jb_autotune_reset(void);
  *   dA (shape of A) = alpha * dC * op(B)^T   (transA 'N')  or  alpha * op(B) * dC^T ('T')
  *   dB (shape of B) = alpha * op(A)^T * dC   (transB 'N')  or  alpha * dC^T * op(A) ('T')
  * Adjoint GEMMs with m*n >= 256 tiles of 128x128 (and k >= 256) run on the
- * single-problem kernels, one after the other on the stream; smaller ones share
- * one grouped launch (transposed-layout loaders).
+ * single-problem kernels, one after the other on the stream (asynchronously);
+ * smaller ones share one one-shot grouped launch (transposed-layout loaders),
+ * which synchronises `stream` like jb_gemm_grouped_dev.
  * dA or dB may be NULL to skip it.  method: JB_F64_NATIVE, JB_F32_NATIVE, JB_F32_BF16X3. */
 int jb_gemm_pullback_dev(int method, char transA, char transB, int64_t m, int64_t n, int64_t k,
                          double alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
@@ -281,6 +309,15 @@ int jb_fill_uniform_dev(uint64_t seed, int64_t n, double* out, void* stream);
 int jb_fill_wide_dev(uint64_t seed, int64_t n, int emin, int emax, double* out, void* stream);
 /* float32 variant of the uniform fill (u rounded to float). */
 int jb_fill_uniform_f32_dev(uint64_t seed, int64_t n, float* out, void* stream);
+/* float32 variant of the wide-range fill: the FP64 value of jb_fill_wide_dev
+ * rounded to float (to nearest).  Bit-identical to the host generator. */
+int jb_fill_wide_f32_dev(uint64_t seed, int64_t n, int emin, int emax, float* out, void* stream);
+/* Normal draws (mrandn's randn(), src/randmat.jl:5-14) by Box-Muller on the same
+ * stream: out[i] = scale * sqrt(-2 ln u1) cos(2 pi u2), u1 = ((x_2i >> 11) + 1) 2^-53,
+ * u2 = (x_2i+1 >> 11) 2^-53.  The device's log and cos make these agree with the
+ * host generator to a few ulp (not bitwise).  The f32 form rounds scale*z to float. */
+int jb_fill_normal_dev(uint64_t seed, int64_t n, double scale, double* out, void* stream);
+int jb_fill_normal_f32_dev(uint64_t seed, int64_t n, double scale, float* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -682,6 +682,10 @@ struct GroupedPlanImpl {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t cap = nullptr;
+  // recorded after every replay: the next replay (on any stream) waits for it,
+  // since all replays share the work and arrival counters
+  hipEvent_t done = nullptr;
+  bool launched = false;
   double flops = 0;
 };
 
@@ -869,6 +873,7 @@ void plan_free(GroupedPlanImpl& P) {
   if (P.exec) hipGraphExecDestroy(P.exec);
   if (P.graph) hipGraphDestroy(P.graph);
   if (P.cap) hipStreamDestroy(P.cap);
+  if (P.done) hipEventDestroy(P.done);
   if (P.dev) hipFree(P.dev);
   P = GroupedPlanImpl();
 }
@@ -885,6 +890,8 @@ int plan_capture(GroupedPlanImpl& P) {
     return fail(JB_ERR_GRAPH, "graph capture failed: %s", hipGetErrorString(le != hipSuccess ? le : e));
   if ((e = hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0)) != hipSuccess)
     return fail(JB_ERR_GRAPH, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  if ((e = hipEventCreateWithFlags(&P.done, hipEventDisableTiming)) != hipSuccess)
+    return hip_fail(e, "grouped plan: event");
   return JB_OK;
 }
 
@@ -1003,9 +1010,17 @@ int jb_grouped_plan_launch(jb_grouped_plan* plan, void* stream) {
   std::lock_guard<std::mutex> lk(rt().mu);
   if (!plan || std::find(g_plans.begin(), g_plans.end(), plan) == g_plans.end())
     return fail(JB_ERR_STATE, "unknown grouped plan");
-  if (plan->impl.nitems == 0) return JB_OK;
-  hipError_t e = hipGraphLaunch(plan->impl.exec, (hipStream_t)stream);
-  return e == hipSuccess ? JB_OK : fail(JB_ERR_GRAPH, "hipGraphLaunch: %s", hipGetErrorString(e));
+  GroupedPlanImpl& P = plan->impl;
+  if (P.nitems == 0) return JB_OK;
+  hipError_t e = hipSuccess;
+  // replays of one plan never overlap: they share its counters
+  if (P.launched && (e = hipStreamWaitEvent((hipStream_t)stream, P.done, 0)) != hipSuccess)
+    return hip_fail(e, "grouped plan: wait for the previous replay");
+  if ((e = hipGraphLaunch(P.exec, (hipStream_t)stream)) != hipSuccess)
+    return fail(JB_ERR_GRAPH, "hipGraphLaunch: %s", hipGetErrorString(e));
+  if ((e = hipEventRecord(P.done, (hipStream_t)stream)) != hipSuccess) return hip_fail(e, "grouped plan: record replay");
+  P.launched = true;
+  return JB_OK;
 }
 
 int jb_grouped_plan_destroy(jb_grouped_plan* plan) {
@@ -1081,6 +1096,35 @@ int jb_autotune_grouped(int method, int64_t count, const jb_gemm_desc* descs, in
   for (auto& kv : buckets)
     if (kv.second.flops >= 0.005 * total) order.push_back({kv.second.flops, kv.first});
   std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  // Trials write into a zeroed scratch copy of every C (ldc = m): the caller's
+  // C is neither read nor written, and beta != 0 problems still read a C.
+  const size_t elt = kind_of(method) == 0 ? 8 : 4;
+  std::vector<jb_gemm_desc> trial(descs, descs + count);
+  size_t cbytes = 0;
+  for (auto& d : trial) {
+    if (d.m <= 0 || d.n <= 0) continue;
+    cbytes = al256(cbytes);
+    d.C = reinterpret_cast<void*>(cbytes);  // offset for now
+    d.ldc = d.m;
+    cbytes += (size_t)d.m * (size_t)d.n * elt;
+  }
+  void* cscratch = nullptr;
+  if (cbytes > 0) {
+    const char* fault = getenv("JB_FAULT_ALLOC");
+    if (fault && *fault == '1') return fail(JB_ERR_ALLOC, "autotune: allocation refused (JB_FAULT_ALLOC)");
+    hipError_t e = hipMalloc(&cscratch, cbytes);
+    if (e != hipSuccess) {
+      hipGetLastError();
+      return fail(JB_ERR_ALLOC, "autotune: hipMalloc(%zu) for the trial outputs failed", cbytes);
+    }
+    if ((e = hipMemsetAsync(cscratch, 0, cbytes, s)) != hipSuccess) {
+      hipFree(cscratch);
+      return hip_fail(e, "autotune: clear trial outputs");
+    }
+    for (auto& d : trial)
+      if (d.m > 0 && d.n > 0) d.C = static_cast<char*>(cscratch) + reinterpret_cast<size_t>(d.C);
+  }
+  descs = trial.data();
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
@@ -1109,6 +1153,10 @@ int jb_autotune_grouped(int method, int64_t count, const jb_gemm_desc* descs, in
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  if (cscratch) {
+    hipStreamSynchronize(s);
+    hipFree(cscratch);
+  }
   if (rc) return rc;
   for (auto& kv : over) g_table[kv.first] = kv.second;
   return JB_OK;

@@ -40,22 +40,33 @@ hipError_t launch_fastmul_f32(int64_t M, int64_t N, int64_t P, float* D, const f
 // Workspace bytes needed for an m x n x k problem with s slices.
 size_t ozaki_workspace_bytes(int64_t m, int64_t n, int64_t k, int s);
 // Split + slice-pair GEMM + compensated combine.  `ws` is device memory of
-// ozaki_workspace_bytes(); `flag` is a device int that the split kernels set
-// to 1 on a non-finite input (caller zeroes it first).
-hipError_t launch_dgemm_ozaki(const GemmArgs& g, int s, void* ws, int* flag, hipStream_t st);
+// ozaki_workspace_bytes().  A row of op(A) or column of op(B) holding Inf or
+// NaN makes the C elements it feeds NaN and sets two device ints to 1: this
+// call's flag, an int at byte offset emulated_flag_offset() of `ws` (cleared
+// by the call itself), and `sticky`, which the call never clears.  No host
+// synchronisation.
+hipError_t launch_dgemm_ozaki(const GemmArgs& g, int s, void* ws, int* sticky, hipStream_t st);
 // The same with INT8 slices (v_mfma_i32_32x32x32_i8, exact int32 level sums).
 size_t ozaki_i8_workspace_bytes(int64_t m, int64_t n, int64_t k, int s);
-hipError_t launch_dgemm_ozaki_i8(const GemmArgs& g, int s, void* ws, int* flag, hipStream_t st);
+hipError_t launch_dgemm_ozaki_i8(const GemmArgs& g, int s, void* ws, int* sticky, hipStream_t st);
+// Byte offset of the call's non-finite flag in the workspace (s = 3, i8 = false
+// for the 3xBF16 SGEMM).
+size_t emulated_flag_offset(int64_t m, int64_t n, int64_t k, int s, bool i8);
 
 // ---- FP32 -----------------------------------------------------------------
 hipError_t launch_sgemm_mfma(const GemmArgs& g, hipStream_t s);
 size_t bf16x3_workspace_bytes(int64_t m, int64_t n, int64_t k);
-hipError_t launch_sgemm_bf16x3(const GemmArgs& g, int nprod, void* ws, hipStream_t s);
+// Non-finite inputs become NaN slices (every product they enter is NaN) and set
+// the two flags as in launch_dgemm_ozaki.
+hipError_t launch_sgemm_bf16x3(const GemmArgs& g, int nprod, void* ws, int* sticky, hipStream_t s);
 
 // ---- generator ------------------------------------------------------------
 hipError_t launch_fill_uniform(uint64_t seed, int64_t n, double* out, hipStream_t s);
 hipError_t launch_fill_uniform_f32(uint64_t seed, int64_t n, float* out, hipStream_t s);
 hipError_t launch_fill_wide(uint64_t seed, int64_t n, int emin, int emax, double* out, hipStream_t s);
+hipError_t launch_fill_wide_f32(uint64_t seed, int64_t n, int emin, int emax, float* out, hipStream_t s);
+hipError_t launch_fill_normal(uint64_t seed, int64_t n, double scale, double* out, hipStream_t s);
+hipError_t launch_fill_normal_f32(uint64_t seed, int64_t n, double scale, float* out, hipStream_t s);
 
 // Compute units of the runtime's device (from the device query at init);
 // launch geometry that depends on the device reads it.

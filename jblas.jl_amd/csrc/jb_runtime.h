@@ -2,6 +2,7 @@
 #pragma once
 #include <functional>
 #include <mutex>
+#include <vector>
 
 #include "../../include/jblas_hip.h"
 #include "jb_internal.h"
@@ -26,7 +27,9 @@ struct Runtime {
   Buffer staging;                // host-pointer entry points: A, B, C copies
   hipEvent_t ws_event = nullptr; // last use of `ws`
   bool ws_pending = false;
-  int* flag = nullptr;           // non-finite input flag of the emulated path
+  bool ws_in_graph = false;      // a graph captured a call that uses `ws`
+  std::vector<void*> retired;    // workspaces that captured graphs may use (freed by jb_finalize)
+  int* sticky = nullptr;         // set by any emulated call that met Inf/NaN; cleared by jb_nonfinite_check
 };
 
 Runtime& rt();
@@ -39,7 +42,6 @@ int init_locked(int device);
 void finalize_locked();
 int ws_acquire(size_t bytes, hipStream_t s, void** out);
 int ws_release(hipStream_t s);
-int device_flag(int** out);
 int run_staged(const GemmArgs& g, size_t elt, bool read_c, const std::function<int(const GemmArgs&, hipStream_t)>& dev);
 
 // One GEMM of a grouped method (JB_F64_NATIVE, JB_F32_NATIVE, JB_F32_BF16X3

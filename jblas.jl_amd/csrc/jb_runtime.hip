@@ -117,6 +117,13 @@ int init_locked(int device) {
     if ((e = hipEventCreateWithFlags(&r.ws_event, hipEventDisableTiming)) != hipSuccess)
       return hip_fail(e, "hipEventCreate");
   }
+  if (r.sticky == nullptr) {
+    if ((e = hipMalloc(&r.sticky, 256)) != hipSuccess) {
+      r.sticky = nullptr;
+      return fail(JB_ERR_ALLOC, "non-finite flag allocation failed: %s", hipGetErrorString(e));
+    }
+    if ((e = hipMemset(r.sticky, 0, 256)) != hipSuccess) return hip_fail(e, "clear the non-finite flag");
+  }
   r.device = device;
   r.inited = true;
   const char* tf = getenv("JB_TUNE_FILE");
@@ -152,10 +159,36 @@ void Buffer::release() {
   size = 0;
 }
 
+static bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
 // Stream-ordered use of the shared workspace: a call waits for the previous
-// user of the workspace (on any stream) before touching it.
+// user of the workspace (on any stream) before touching it.  While `s` is being
+// captured into a graph the workspace must already be large enough (a graph
+// keeps its address; growing it would free memory that the graph uses), and
+// the captured work is not ordered against eager calls: the caller orders the
+// graph's replays with other calls of the library.
 int ws_acquire(size_t bytes, hipStream_t s, void** out) {
   Runtime& r = rt();
+  if (capturing(s)) {
+    if (bytes > r.ws.size)
+      return fail(JB_ERR_STATE,
+                  "the workspace cannot grow during graph capture: make one call of this size (%zu bytes of "
+                  "workspace) outside the capture first",
+                  bytes);
+    r.ws_in_graph = true;
+    *out = r.ws.ptr;
+    return JB_OK;
+  }
+  if (bytes > r.ws.size && r.ws_in_graph) {
+    // a captured graph refers to the current buffer: keep it until jb_finalize
+    r.retired.push_back(r.ws.ptr);
+    r.ws.ptr = nullptr;
+    r.ws.size = 0;
+    r.ws_in_graph = false;
+  }
   int rc = r.ws.reserve(bytes, "workspace");
   if (rc) return rc;
   if (r.ws_pending) {
@@ -168,6 +201,7 @@ int ws_acquire(size_t bytes, hipStream_t s, void** out) {
 
 int ws_release(hipStream_t s) {
   Runtime& r = rt();
+  if (capturing(s)) return JB_OK;
   hipError_t e = hipEventRecord(r.ws_event, s);
   if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
   r.ws_pending = true;
@@ -197,13 +231,16 @@ static size_t span_bytes(int64_t rows, int64_t cols, int64_t ld, size_t elt) {
 }
 static size_t round_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Runs `dev` (a device-pointer GEMM) on staged copies of host operands.
+// Runs `dev` (a device-pointer GEMM) on staged copies of host operands.  With
+// alpha == 0 or k == 0 the device GEMM reads neither A nor B (C = beta*C), so
+// they are not staged: the caller may pass NULL or a short buffer then.
 int run_staged(const GemmArgs& g, size_t elt, bool read_c, const std::function<int(const GemmArgs&, hipStream_t)>& dev) {
   Runtime& r = rt();
+  const bool uses_ab = g.alpha != 0.0 && g.k > 0;
   const int64_t arows = g.ta ? g.k : g.m, acols = g.ta ? g.m : g.k;
   const int64_t brows = g.tb ? g.n : g.k, bcols = g.tb ? g.k : g.n;
-  const size_t sa = round_up(span_bytes(arows, acols, g.lda, elt));
-  const size_t sb = round_up(span_bytes(brows, bcols, g.ldb, elt));
+  const size_t sa = uses_ab ? round_up(span_bytes(arows, acols, g.lda, elt)) : 0;
+  const size_t sb = uses_ab ? round_up(span_bytes(brows, bcols, g.ldb, elt)) : 0;
   const size_t sc = round_up(span_bytes(g.m, g.n, g.ldc, elt));
   int rc = r.staging.reserve(sa + sb + sc + 256, "staging");
   if (rc) return rc;
@@ -214,8 +251,9 @@ int run_staged(const GemmArgs& g, size_t elt, bool read_c, const std::function<i
   d.C = base + sa + sb;
   hipStream_t s = r.stream;
   hipError_t e;
-  if ((e = h2d(base, g.A, arows, acols, g.lda, elt, s)) != hipSuccess) return hip_fail(e, "copy A to device");
-  if ((e = h2d(base + sa, g.B, brows, bcols, g.ldb, elt, s)) != hipSuccess) return hip_fail(e, "copy B to device");
+  if (uses_ab && (e = h2d(base, g.A, arows, acols, g.lda, elt, s)) != hipSuccess) return hip_fail(e, "copy A to device");
+  if (uses_ab && (e = h2d(base + sa, g.B, brows, bcols, g.ldb, elt, s)) != hipSuccess)
+    return hip_fail(e, "copy B to device");
   if (read_c && (e = h2d(d.C, g.C, g.m, g.n, g.ldc, elt, s)) != hipSuccess) return hip_fail(e, "copy C to device");
   rc = dev(d, s);
   if (rc) {
@@ -233,10 +271,13 @@ void finalize_locked() {
   hipDeviceSynchronize();
   grouped_release_all();
   r.ws.release();
+  for (void* p : r.retired) hipFree(p);
+  r.retired.clear();
+  r.ws_in_graph = false;
   r.staging.release();
-  if (r.flag) {
-    hipFree(r.flag);
-    r.flag = nullptr;
+  if (r.sticky) {
+    hipFree(r.sticky);
+    r.sticky = nullptr;
   }
   if (r.ws_event) hipEventDestroy(r.ws_event);
   r.ws_event = nullptr;
@@ -254,11 +295,6 @@ struct TimingRec {
 static bool g_timing = false;
 static std::vector<TimingRec> g_pending;
 static std::map<std::string, std::pair<double, int64_t>> g_totals;
-
-static bool capturing(hipStream_t s) {
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
-}
 
 int runtime_compute_units() { return rt().info.compute_units; }
 
@@ -297,18 +333,6 @@ static void timing_collect() {
   g_pending.clear();
 }
 
-int device_flag(int** out) {
-  Runtime& r = rt();
-  if (!r.flag) {
-    hipError_t e = hipMalloc(&r.flag, 256);
-    if (e != hipSuccess) {
-      r.flag = nullptr;
-      return fail(JB_ERR_ALLOC, "flag allocation failed: %s", hipGetErrorString(e));
-    }
-  }
-  *out = r.flag;
-  return JB_OK;
-}
 
 }  // namespace jb
 
@@ -580,31 +604,31 @@ int jb_fastmul_f32_dev(int64_t M, int64_t N, int64_t P, float* D, const float* A
 }
 
 // ---- FP64 emulated (split-BF16 / split-INT8) -------------------------------
-static int ozaki_dev_locked(const GemmArgs& g, int s, bool i8, hipStream_t st) {
+// Enqueues the emulated GEMM on `st` without synchronising.  A non-finite
+// operand makes the C elements it feeds NaN and raises the sticky flag
+// (jb_nonfinite_check); with `host_flag` the call's own flag is also copied to
+// *host_flag in stream order (the host-pointer entry points, which synchronise).
+static int ozaki_dev_locked(const GemmArgs& g, int s, bool i8, hipStream_t st, int* host_flag = nullptr) {
   if (g.m == 0 || g.n == 0) return JB_OK;
   if (g.alpha == 0.0 || g.k == 0) {
     hipError_t e = launch_scale_f64(g.m, g.n, g.beta, static_cast<double*>(g.C), g.ldc, st);
     return e == hipSuccess ? JB_OK : hip_fail(e, "scale launch");
   }
-  int* flag = nullptr;
-  int rc = device_flag(&flag);
-  if (rc) return rc;
   void* ws = nullptr;
-  rc = ws_acquire(i8 ? ozaki_i8_workspace_bytes(g.m, g.n, g.k, s) : ozaki_workspace_bytes(g.m, g.n, g.k, s), st, &ws);
+  int rc = ws_acquire(i8 ? ozaki_i8_workspace_bytes(g.m, g.n, g.k, s) : ozaki_workspace_bytes(g.m, g.n, g.k, s), st, &ws);
   if (rc) return rc;
-  hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), st);
-  if (e == hipSuccess) e = i8 ? launch_dgemm_ozaki_i8(g, s, ws, flag, st) : launch_dgemm_ozaki(g, s, ws, flag, st);
+  hipError_t e = i8 ? launch_dgemm_ozaki_i8(g, s, ws, rt().sticky, st) : launch_dgemm_ozaki(g, s, ws, rt().sticky, st);
+  if (e == hipSuccess && host_flag)
+    e = hipMemcpyAsync(host_flag, static_cast<char*>(ws) + emulated_flag_offset(g.m, g.n, g.k, s, i8), sizeof(int),
+                       hipMemcpyDeviceToHost, st);
   int relrc = ws_release(st);
   if (e != hipSuccess) return hip_fail(e, "dgemm_ozaki launch");
-  if (relrc) return relrc;
-  // The non-finite check needs the flag on the host: this is the one
-  // synchronisation point of the emulated path.
-  int h = 0;
-  e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  if (e != hipSuccess) return hip_fail(e, "non-finite flag readback");
-  if (h) return fail(JB_ERR_NONFINITE, "an operand of the emulated DGEMM holds Inf or NaN; C is unspecified");
-  return JB_OK;
+  return relrc;
+}
+
+static int nonfinite_host(int flag, const char* what) {
+  if (!flag) return JB_OK;
+  return fail(JB_ERR_NONFINITE, "an operand of the %s holds Inf or NaN: the C elements it feeds are NaN", what);
 }
 
 int jb_dgemm_ozaki_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
@@ -619,9 +643,11 @@ int jb_dgemm_ozaki(char transA, char transB, int64_t m, int64_t n, int64_t k, do
                    int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int nslices) {
   if (nslices < 1 || nslices > 8) return fail(-14, "nslices must be in 1..8 (got %d)", nslices);
   JB_GEMM_PROLOGUE(alpha, beta)
-  return run_staged(g, sizeof(double), beta != 0.0, [nslices](const GemmArgs& d, hipStream_t s) {
-    return ozaki_dev_locked(d, nslices, false, s);
+  int bad = 0;
+  rc = run_staged(g, sizeof(double), beta != 0.0, [nslices, &bad](const GemmArgs& d, hipStream_t s) {
+    return ozaki_dev_locked(d, nslices, false, s, &bad);
   });
+  return rc ? rc : nonfinite_host(bad, "emulated DGEMM");
 }
 
 int jb_dgemm_ozaki_i8_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
@@ -636,13 +662,15 @@ int jb_dgemm_ozaki_i8(char transA, char transB, int64_t m, int64_t n, int64_t k,
                       int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int nslices) {
   if (nslices < 1 || nslices > 8) return fail(-14, "nslices must be in 1..8 (got %d)", nslices);
   JB_GEMM_PROLOGUE(alpha, beta)
-  return run_staged(g, sizeof(double), beta != 0.0, [nslices](const GemmArgs& d, hipStream_t s) {
-    return ozaki_dev_locked(d, nslices, true, s);
+  int bad = 0;
+  rc = run_staged(g, sizeof(double), beta != 0.0, [nslices, &bad](const GemmArgs& d, hipStream_t s) {
+    return ozaki_dev_locked(d, nslices, true, s, &bad);
   });
+  return rc ? rc : nonfinite_host(bad, "emulated DGEMM");
 }
 
 // ---- FP32 -------------------------------------------------------------------
-static int sgemm_dev_locked(const GemmArgs& g, int method, int nprod, hipStream_t s) {
+static int sgemm_dev_locked(const GemmArgs& g, int method, int nprod, hipStream_t s, int* host_flag = nullptr) {
   if (g.m == 0 || g.n == 0) return JB_OK;
   hipError_t e;
   if (g.alpha == 0.0 || g.k == 0) {
@@ -656,7 +684,10 @@ static int sgemm_dev_locked(const GemmArgs& g, int method, int nprod, hipStream_
   void* ws = nullptr;
   int rc = ws_acquire(bf16x3_workspace_bytes(g.m, g.n, g.k), s, &ws);
   if (rc) return rc;
-  e = launch_sgemm_bf16x3(g, nprod, ws, s);
+  e = launch_sgemm_bf16x3(g, nprod, ws, rt().sticky, s);
+  if (e == hipSuccess && host_flag)
+    e = hipMemcpyAsync(host_flag, static_cast<char*>(ws) + emulated_flag_offset(g.m, g.n, g.k, 3, false), sizeof(int),
+                       hipMemcpyDeviceToHost, s);
   rc = ws_release(s);
   if (e != hipSuccess) return hip_fail(e, "sgemm_bf16x3 launch");
   return rc;
@@ -697,9 +728,27 @@ int jb_sgemm_bf16x3(char transA, char transB, int64_t m, int64_t n, int64_t k, f
                     int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int nprod) {
   if (nprod != 3 && nprod != 6 && nprod != 9) return fail(-14, "nprod must be 3, 6 or 9 (got %d)", nprod);
   JB_GEMM_PROLOGUE(alpha, beta)
-  return run_staged(g, sizeof(float), beta != 0.0f, [nprod](const GemmArgs& d, hipStream_t s) {
-    return sgemm_dev_locked(d, JB_F32_BF16X3, nprod, s);
+  int bad = 0;
+  rc = run_staged(g, sizeof(float), beta != 0.0f, [nprod, &bad](const GemmArgs& d, hipStream_t s) {
+    return sgemm_dev_locked(d, JB_F32_BF16X3, nprod, s, &bad);
   });
+  return rc ? rc : nonfinite_host(bad, "3xBF16 SGEMM");
+}
+
+int jb_nonfinite_check(void* stream) {
+  std::lock_guard<std::mutex> lk(rt().mu);
+  tl_error.clear();
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (capturing(s)) return fail(JB_ERR_STATE, "jb_nonfinite_check synchronises: not allowed during graph capture");
+  int h = 0;
+  hipError_t e = hipMemcpyAsync(&h, rt().sticky, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess && h) e = hipMemsetAsync(rt().sticky, 0, sizeof(int), s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "non-finite flag readback");
+  return h ? fail(JB_ERR_NONFINITE, "an emulated call since the last check met Inf or NaN in an operand") : JB_OK;
 }
 
 // ---- generator --------------------------------------------------------------
@@ -725,6 +774,51 @@ int jb_fill_uniform_f32_dev(uint64_t seed, int64_t n, float* out, void* stream) 
   if (rc) return rc;
   hipError_t e = launch_fill_uniform_f32(seed, n, out, (hipStream_t)stream);
   return e == hipSuccess ? JB_OK : hip_fail(e, "fill launch");
+}
+
+extern "C++" {
+template <typename T>
+static int fill_wide_entry(uint64_t seed, int64_t n, int emin, int emax, T* out, void* stream) {
+  tl_error.clear();
+  if (n < 0) return fail(-2, "n < 0");
+  if (emin > emax) return fail(-4, "emax < emin");
+  if (emin < -1000 || emax > 1000) return fail(-3, "exponent range outside [-1000, 1000]");
+  if (n > 0 && !out) return fail(-5, "out is NULL");
+  std::lock_guard<std::mutex> lk(rt().mu);
+  if (n == 0) return JB_OK;
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipError_t e = sizeof(T) == 8 ? launch_fill_wide(seed, n, emin, emax, (double*)out, (hipStream_t)stream)
+                                : launch_fill_wide_f32(seed, n, emin, emax, (float*)out, (hipStream_t)stream);
+  return e == hipSuccess ? JB_OK : hip_fail(e, "fill launch");
+}
+
+template <typename T>
+static int fill_normal_entry(uint64_t seed, int64_t n, double scale, T* out, void* stream) {
+  tl_error.clear();
+  if (n < 0) return fail(-2, "n < 0");
+  if (!(scale == scale) || scale - scale != 0.0) return fail(-3, "scale is not finite");
+  if (n > 0 && !out) return fail(-4, "out is NULL");
+  std::lock_guard<std::mutex> lk(rt().mu);
+  if (n == 0) return JB_OK;
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipError_t e = sizeof(T) == 8 ? launch_fill_normal(seed, n, scale, (double*)out, (hipStream_t)stream)
+                                : launch_fill_normal_f32(seed, n, scale, (float*)out, (hipStream_t)stream);
+  return e == hipSuccess ? JB_OK : hip_fail(e, "fill launch");
+}
+}  // extern "C++"
+
+int jb_fill_wide_f32_dev(uint64_t seed, int64_t n, int emin, int emax, float* out, void* stream) {
+  return fill_wide_entry<float>(seed, n, emin, emax, out, stream);
+}
+
+int jb_fill_normal_dev(uint64_t seed, int64_t n, double scale, double* out, void* stream) {
+  return fill_normal_entry<double>(seed, n, scale, out, stream);
+}
+
+int jb_fill_normal_f32_dev(uint64_t seed, int64_t n, double scale, float* out, void* stream) {
+  return fill_normal_entry<float>(seed, n, scale, out, stream);
 }
 
 int jb_fill_wide_dev(uint64_t seed, int64_t n, int emin, int emax, double* out, void* stream) {

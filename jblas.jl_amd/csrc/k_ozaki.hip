@@ -76,6 +76,10 @@ __host__ __device__ constexpr int level_chunk(int L) {
   return kc;
 }
 
+// Exponent word of a row of op(A) / column of op(B) that holds Inf or NaN: the
+// epilogue stores NaN in every element of C that such a row or column feeds.
+constexpr int EXP_NONFINITE = 0x40000000;
+
 // Element offset of (row, k) in one tile-contiguous plane with row block RB.
 __device__ __forceinline__ int64_t plane_off(int64_t row, int64_t k, int RB, int64_t Kpad) {
   return (row / RB) * (RB * Kpad) + (k / QBK) * (RB * QBK) + (row % RB) * QBK + (k % QBK);
@@ -180,7 +184,7 @@ __global__ __launch_bounds__(256) void split_f64_kernel(const double* __restrict
                                                         int64_t K, const unsigned long long* __restrict__ maxbits,
                                                         int* __restrict__ exps, uint16_t* __restrict__ planes,
                                                         int64_t plane_stride, int64_t Kpad, int RB,
-                                                        int* __restrict__ flag) {
+                                                        int* __restrict__ call_flag, int* __restrict__ sticky) {
   __shared__ double tile[64][65];
   const int t = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
@@ -209,12 +213,14 @@ __global__ __launch_bounds__(256) void split_f64_kernel(const double* __restrict
     int E = 0;
     if (r0 + row < R) {
       const unsigned long long mb = maxbits[r0 + row];
-      if (mb >= 0x7FF0000000000000ull) {
-        *flag = 1;  // Inf or NaN in this row/column
+      const bool bad = mb >= 0x7FF0000000000000ull;  // Inf or NaN in this row/column
+      if (bad) {
+        *call_flag = 1;
+        *sticky = 1;
       } else {
         E = exponent_of_bits(mb);
       }
-      if (blockIdx.y == 0 && kg == 0) exps[r0 + row] = E;
+      if (blockIdx.y == 0 && kg == 0) exps[r0 + row] = bad ? EXP_NONFINITE : E;
     }
     double res[8];
 #pragma unroll
@@ -258,7 +264,7 @@ __global__ __launch_bounds__(256) void split_f64_i8_kernel(const double* __restr
                                                            const unsigned long long* __restrict__ maxbits,
                                                            int* __restrict__ exps, uint8_t* __restrict__ planes,
                                                            int64_t plane_stride, int64_t Kpad, int RB,
-                                                           int* __restrict__ flag) {
+                                                           int* __restrict__ call_flag, int* __restrict__ sticky) {
   __shared__ double tile[64][65];
   const int t = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
@@ -287,13 +293,15 @@ __global__ __launch_bounds__(256) void split_f64_i8_kernel(const double* __restr
     int E = 0;
     if (r0 + row < R) {
       const unsigned long long mb = maxbits[r0 + row];
-      if (mb >= 0x7FF0000000000000ull) {
-        *flag = 1;  // Inf or NaN in this row/column
+      const bool bad = mb >= 0x7FF0000000000000ull;  // Inf or NaN in this row/column
+      if (bad) {
+        *call_flag = 1;
+        *sticky = 1;
       } else if (mb != 0ull) {
         E = exponent_of_bits(mb);
         if (ldexp(__longlong_as_double((long long)mb), -E) > 126.5 / 128.0) E += 1;
       }
-      if (blockIdx.y == 0 && kg == 0) exps[r0 + row] = E;
+      if (blockIdx.y == 0 && kg == 0) exps[r0 + row] = bad ? EXP_NONFINITE : E;
     }
     long long X[8];
 #pragma unroll
@@ -325,7 +333,8 @@ __device__ __forceinline__ float bf16_to_f(uint16_t b) { return __uint_as_float(
 
 __global__ __launch_bounds__(256) void split_f32_kernel(const float* __restrict__ P, int64_t sr, int64_t sk, int64_t R,
                                                         int64_t K, uint16_t* __restrict__ planes, int64_t plane_stride,
-                                                        int64_t Kpad, int RB) {
+                                                        int64_t Kpad, int RB, int* __restrict__ call_flag,
+                                                        int* __restrict__ sticky) {
   __shared__ float tile[64][65];
   const int t = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
@@ -352,20 +361,35 @@ __global__ __launch_bounds__(256) void split_f32_kernel(const float* __restrict_
     const int grp = t + 256 * q;
     const int row = grp / 8, kg = 8 * (grp % 8);
     uint32_t w[3][4];
+    bool bad = false;
 #pragma unroll
     for (int e = 0; e < 8; e += 2) {
       uint16_t b[3][2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         float r = tile[row][kg + e + h];
+        if (!isfinite(r)) {
+          // Inf or NaN: one NaN slice, so that every product it enters is NaN
+          bad = true;
+          b[0][h] = 0x7FC0;
+          b[1][h] = b[2][h] = 0;
+          continue;
+        }
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
           b[s][h] = bf16_rn_bits(r);
+          // |a| within half a BF16 ulp of FLT_MAX rounds to Inf: truncate instead
+          // (the residual then has at most 16 significant bits, still exact)
+          if (s == 0 && (b[s][h] & 0x7F80u) == 0x7F80u) b[s][h] = (uint16_t)(__float_as_uint(r) >> 16);
           r = r - bf16_to_f(b[s][h]);
         }
       }
 #pragma unroll
       for (int s = 0; s < 3; ++s) w[s][e / 2] = (uint32_t)b[s][0] | ((uint32_t)b[s][1] << 16);
+    }
+    if (bad) {
+      *call_flag = 1;
+      *sticky = 1;
     }
     const int64_t off = plane_off(r0 + row, k0 + kg, RB, Kpad);
 #pragma unroll
@@ -650,6 +674,7 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
       for (int e = 0; e < 16; ++e) {
         const int64_t n = n0 + wn0 + 8 * (e / 4) + 4 * hl + (e % 4);
         if (n < eN) {
+          const int eb = p.eB[n];
           double hi = 0.0, lo = 0.0;
 #pragma unroll
           for (int g = NG - 1; g >= 0; --g) {
@@ -657,9 +682,10 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
             two_sum(hi, ldexp(G[g][b][e], -DB * GL * g), hi, err);
             lo += err;
           }
-          const double v = ldexp(hi + lo, ea + p.eB[n] - (MODE == 2 ? 14 : 16));
+          const double v = ldexp(hi + lo, ea + eb - (MODE == 2 ? 14 : 16));
           double c = ealpha * v;
           if (ebeta != 0.0) c = fma(ebeta, C[m + n * eldc], c);
+          if (ea == EXP_NONFINITE || eb == EXP_NONFINITE) c = __builtin_nan("");
           C[m + n * eldc] = c;
         }
       }
@@ -691,7 +717,7 @@ inline int64_t rup(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 struct Layout {
   int64_t Mpad, Npad, Kpad;
-  size_t offA, offB, offMA, offMB, offEA, offEB, total;
+  size_t offA, offB, offMA, offMB, offEA, offEB, offFlag, total;
 };
 
 // elt: bytes per digit (2: BF16 planes, 1: INT8 planes).
@@ -714,6 +740,8 @@ Layout make_layout(int64_t m, int64_t n, int64_t k, int s, int elt = 2) {
   off = al(off + (size_t)L.Mpad * 4);
   L.offEB = off;
   off = al(off + (size_t)L.Npad * 4);
+  L.offFlag = off;  // this call's non-finite flag
+  off = al(off + 8);
   L.total = off;
   return L;
 }
@@ -731,12 +759,19 @@ inline void strides_b(bool tb, int64_t ldb, int64_t& sr, int64_t& sk) {
 
 template <int S>
 void launch_split_f64(const double* P, int64_t sr, int64_t sk, int64_t R, int64_t K, int64_t Rpad, int64_t Kpad, int RB,
-                      unsigned long long* mb, int* ex, uint16_t* planes, int* flag, hipStream_t st) {
+                      unsigned long long* mb, int* ex, uint16_t* planes, int* call_flag, int* sticky, hipStream_t st) {
   hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)std::min<int64_t>((Rpad + 255) / 256, 1024)), dim3(256), 0, st,
                      mb, Rpad);
   launch_absmax(P, sr, sk, R, K, mb, st);
   hipLaunchKernelGGL(split_f64_kernel<S>, dim3((unsigned)(Rpad / 64), (unsigned)(Kpad / 64)), dim3(256), 0, st, P, sr,
-                     sk, R, K, mb, ex, planes, Rpad * Kpad, Kpad, RB, flag);
+                     sk, R, K, mb, ex, planes, Rpad * Kpad, Kpad, RB, call_flag, sticky);
+}
+
+// Clears this call's non-finite flag (a kernel, not a memset node, so that the
+// call can be captured into a graph).
+void clear_call_flag(int* call_flag, hipStream_t st) {
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned long long*>(call_flag),
+                     (int64_t)1);
 }
 
 // elt as in make_layout: the kernel addresses planes in 16-bit units, so an
@@ -763,8 +798,9 @@ PairArgs pair_args(const GemmArgs& g, const Layout& L, const uint16_t* pA, const
 }
 
 template <int S>
-hipError_t run_ozaki(const GemmArgs& g, const Layout& L, char* ws, int* flag, hipStream_t st) {
+hipError_t run_ozaki(const GemmArgs& g, const Layout& L, char* ws, int* sticky, hipStream_t st) {
   int64_t sr, sk;
+  int* flag = reinterpret_cast<int*>(ws + L.offFlag);
   auto* mbA = reinterpret_cast<unsigned long long*>(ws + L.offMA);
   auto* mbB = reinterpret_cast<unsigned long long*>(ws + L.offMB);
   auto* eA = reinterpret_cast<int*>(ws + L.offEA);
@@ -772,10 +808,13 @@ hipError_t run_ozaki(const GemmArgs& g, const Layout& L, char* ws, int* flag, hi
   auto* pA = reinterpret_cast<uint16_t*>(ws + L.offA);
   auto* pB = reinterpret_cast<uint16_t*>(ws + L.offB);
   timing_begin(st, "ozaki_split");
+  clear_call_flag(flag, st);
   strides_a(g.ta, g.lda, sr, sk);
-  launch_split_f64<S>(static_cast<const double*>(g.A), sr, sk, g.m, g.k, L.Mpad, L.Kpad, QBM, mbA, eA, pA, flag, st);
+  launch_split_f64<S>(static_cast<const double*>(g.A), sr, sk, g.m, g.k, L.Mpad, L.Kpad, QBM, mbA, eA, pA, flag, sticky,
+                      st);
   strides_b(g.tb, g.ldb, sr, sk);
-  launch_split_f64<S>(static_cast<const double*>(g.B), sr, sk, g.n, g.k, L.Npad, L.Kpad, QBN, mbB, eB, pB, flag, st);
+  launch_split_f64<S>(static_cast<const double*>(g.B), sr, sk, g.n, g.k, L.Npad, L.Kpad, QBN, mbB, eB, pB, flag, sticky,
+                      st);
   timing_end(st, "ozaki_split");
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -788,17 +827,19 @@ hipError_t run_ozaki(const GemmArgs& g, const Layout& L, char* ws, int* flag, hi
 
 template <int S>
 void launch_split_f64_i8(const double* P, int64_t sr, int64_t sk, int64_t R, int64_t K, int64_t Rpad, int64_t Kpad,
-                         int RB, unsigned long long* mb, int* ex, uint8_t* planes, int* flag, hipStream_t st) {
+                         int RB, unsigned long long* mb, int* ex, uint8_t* planes, int* call_flag, int* sticky,
+                         hipStream_t st) {
   hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)std::min<int64_t>((Rpad + 255) / 256, 1024)), dim3(256), 0, st,
                      mb, Rpad);
   launch_absmax(P, sr, sk, R, K, mb, st);
   hipLaunchKernelGGL(split_f64_i8_kernel<S>, dim3((unsigned)(Rpad / 64), (unsigned)(Kpad / 64)), dim3(256), 0, st, P,
-                     sr, sk, R, K, mb, ex, planes, Rpad * Kpad, Kpad, RB, flag);
+                     sr, sk, R, K, mb, ex, planes, Rpad * Kpad, Kpad, RB, call_flag, sticky);
 }
 
 template <int S>
-hipError_t run_ozaki_i8(const GemmArgs& g, const Layout& L, char* ws, int* flag, hipStream_t st) {
+hipError_t run_ozaki_i8(const GemmArgs& g, const Layout& L, char* ws, int* sticky, hipStream_t st) {
   int64_t sr, sk;
+  int* flag = reinterpret_cast<int*>(ws + L.offFlag);
   auto* mbA = reinterpret_cast<unsigned long long*>(ws + L.offMA);
   auto* mbB = reinterpret_cast<unsigned long long*>(ws + L.offMB);
   auto* eA = reinterpret_cast<int*>(ws + L.offEA);
@@ -806,12 +847,13 @@ hipError_t run_ozaki_i8(const GemmArgs& g, const Layout& L, char* ws, int* flag,
   auto* pA = reinterpret_cast<uint8_t*>(ws + L.offA);
   auto* pB = reinterpret_cast<uint8_t*>(ws + L.offB);
   timing_begin(st, "ozaki_i8_split");
+  clear_call_flag(flag, st);
   strides_a(g.ta, g.lda, sr, sk);
   launch_split_f64_i8<S>(static_cast<const double*>(g.A), sr, sk, g.m, g.k, L.Mpad, L.Kpad, QBM, mbA, eA, pA, flag,
-                         st);
+                         sticky, st);
   strides_b(g.tb, g.ldb, sr, sk);
   launch_split_f64_i8<S>(static_cast<const double*>(g.B), sr, sk, g.n, g.k, L.Npad, L.Kpad, QBN, mbB, eB, pB, flag,
-                         st);
+                         sticky, st);
   timing_end(st, "ozaki_i8_split");
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -828,55 +870,61 @@ hipError_t run_ozaki_i8(const GemmArgs& g, const Layout& L, char* ws, int* flag,
 
 size_t ozaki_workspace_bytes(int64_t m, int64_t n, int64_t k, int s) { return make_layout(m, n, k, s).total; }
 
+size_t emulated_flag_offset(int64_t m, int64_t n, int64_t k, int s, bool i8) {
+  return make_layout(m, n, k, s, i8 ? 1 : 2).offFlag;
+}
+
 size_t ozaki_i8_workspace_bytes(int64_t m, int64_t n, int64_t k, int s) { return make_layout(m, n, k, s, 1).total; }
 
-hipError_t launch_dgemm_ozaki_i8(const GemmArgs& g, int s, void* ws, int* flag, hipStream_t st) {
+hipError_t launch_dgemm_ozaki_i8(const GemmArgs& g, int s, void* ws, int* sticky, hipStream_t st) {
   const Layout L = make_layout(g.m, g.n, g.k, s, 1);
   char* w = static_cast<char*>(ws);
   switch (s) {
-    case 1: return run_ozaki_i8<1>(g, L, w, flag, st);
-    case 2: return run_ozaki_i8<2>(g, L, w, flag, st);
-    case 3: return run_ozaki_i8<3>(g, L, w, flag, st);
-    case 4: return run_ozaki_i8<4>(g, L, w, flag, st);
-    case 5: return run_ozaki_i8<5>(g, L, w, flag, st);
-    case 6: return run_ozaki_i8<6>(g, L, w, flag, st);
-    case 7: return run_ozaki_i8<7>(g, L, w, flag, st);
-    case 8: return run_ozaki_i8<8>(g, L, w, flag, st);
+    case 1: return run_ozaki_i8<1>(g, L, w, sticky, st);
+    case 2: return run_ozaki_i8<2>(g, L, w, sticky, st);
+    case 3: return run_ozaki_i8<3>(g, L, w, sticky, st);
+    case 4: return run_ozaki_i8<4>(g, L, w, sticky, st);
+    case 5: return run_ozaki_i8<5>(g, L, w, sticky, st);
+    case 6: return run_ozaki_i8<6>(g, L, w, sticky, st);
+    case 7: return run_ozaki_i8<7>(g, L, w, sticky, st);
+    case 8: return run_ozaki_i8<8>(g, L, w, sticky, st);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_dgemm_ozaki(const GemmArgs& g, int s, void* ws, int* flag, hipStream_t st) {
+hipError_t launch_dgemm_ozaki(const GemmArgs& g, int s, void* ws, int* sticky, hipStream_t st) {
   const Layout L = make_layout(g.m, g.n, g.k, s);
   char* w = static_cast<char*>(ws);
   switch (s) {
-    case 1: return run_ozaki<1>(g, L, w, flag, st);
-    case 2: return run_ozaki<2>(g, L, w, flag, st);
-    case 3: return run_ozaki<3>(g, L, w, flag, st);
-    case 4: return run_ozaki<4>(g, L, w, flag, st);
-    case 5: return run_ozaki<5>(g, L, w, flag, st);
-    case 6: return run_ozaki<6>(g, L, w, flag, st);
-    case 7: return run_ozaki<7>(g, L, w, flag, st);
-    case 8: return run_ozaki<8>(g, L, w, flag, st);
+    case 1: return run_ozaki<1>(g, L, w, sticky, st);
+    case 2: return run_ozaki<2>(g, L, w, sticky, st);
+    case 3: return run_ozaki<3>(g, L, w, sticky, st);
+    case 4: return run_ozaki<4>(g, L, w, sticky, st);
+    case 5: return run_ozaki<5>(g, L, w, sticky, st);
+    case 6: return run_ozaki<6>(g, L, w, sticky, st);
+    case 7: return run_ozaki<7>(g, L, w, sticky, st);
+    case 8: return run_ozaki<8>(g, L, w, sticky, st);
     default: return hipErrorInvalidValue;
   }
 }
 
 size_t bf16x3_workspace_bytes(int64_t m, int64_t n, int64_t k) { return make_layout(m, n, k, 3).total; }
 
-hipError_t launch_sgemm_bf16x3(const GemmArgs& g, int nprod, void* ws, hipStream_t st) {
+hipError_t launch_sgemm_bf16x3(const GemmArgs& g, int nprod, void* ws, int* sticky, hipStream_t st) {
   const Layout L = make_layout(g.m, g.n, g.k, 3);
   char* w = static_cast<char*>(ws);
   auto* pA = reinterpret_cast<uint16_t*>(w + L.offA);
   auto* pB = reinterpret_cast<uint16_t*>(w + L.offB);
+  int* flag = reinterpret_cast<int*>(w + L.offFlag);
   int64_t sr, sk;
   timing_begin(st, "bf16x3_split");
+  clear_call_flag(flag, st);
   strides_a(g.ta, g.lda, sr, sk);
   hipLaunchKernelGGL(split_f32_kernel, dim3((unsigned)(L.Mpad / 64), (unsigned)(L.Kpad / 64)), dim3(256), 0, st,
-                     static_cast<const float*>(g.A), sr, sk, g.m, g.k, pA, L.Mpad * L.Kpad, L.Kpad, QBM);
+                     static_cast<const float*>(g.A), sr, sk, g.m, g.k, pA, L.Mpad * L.Kpad, L.Kpad, QBM, flag, sticky);
   strides_b(g.tb, g.ldb, sr, sk);
   hipLaunchKernelGGL(split_f32_kernel, dim3((unsigned)(L.Npad / 64), (unsigned)(L.Kpad / 64)), dim3(256), 0, st,
-                     static_cast<const float*>(g.B), sr, sk, g.n, g.k, pB, L.Npad * L.Kpad, L.Kpad, QBN);
+                     static_cast<const float*>(g.B), sr, sk, g.n, g.k, pB, L.Npad * L.Kpad, L.Kpad, QBN, flag, sticky);
   timing_end(st, "bf16x3_split");
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -7,15 +7,17 @@ gemm!, grouped launches, the autotuner and the gemm! pullback.
 """
 from ._lib import (JB_F32_BF16X3, JB_F32_NATIVE, JB_F64_NATIVE, JB_F64_OZAKI_BASE, JB_F64_RECURRENCE, JBError,
                    device_info, finalize, init, kernel_time, kernel_timing, kernel_timing_reset, last_error, lib)
-from .api import (DimensionMismatch, device_matrix, fill_uniform_, fill_wide_, gemm_, initkernel_, jmul_, fastmul_,
-                  kernel_)
+from .api import (DimensionMismatch, device_matrix, fill_normal_, fill_uniform_, fill_wide_, gemm_, initkernel_, jmul_, fastmul_,
+                  kernel_, nonfinite_check)
 from .grouped import (GemmDesc, GroupedPlan, autotune, gemm_grouped_, split_chunk, tune_load, tune_lookup,
                       tune_reset, tune_save)
 
 __all__ = [
     "JBError", "DimensionMismatch", "lib", "init", "finalize", "last_error", "device_info",
     "kernel_timing", "kernel_time", "kernel_timing_reset",
-    "gemm_", "jmul_", "fastmul_", "kernel_", "initkernel_", "fill_uniform_", "fill_wide_", "device_matrix",
+    "gemm_", "jmul_", "fastmul_", "kernel_", "initkernel_", "fill_uniform_", "fill_wide_", "fill_normal_",
+    "device_matrix",
+    "nonfinite_check",
     "GemmDesc", "GroupedPlan", "gemm_grouped_", "autotune", "tune_save", "tune_load", "tune_reset",
     "tune_lookup", "split_chunk",
     "JB_F64_NATIVE", "JB_F64_RECURRENCE", "JB_F32_NATIVE", "JB_F32_BF16X3", "JB_F64_OZAKI_BASE",

@@ -90,6 +90,7 @@ SIGNATURES = {
     "jb_sgemm_dev": (_int, [_ch, _ch, _i64, _i64, _i64, _flt, _p, _i64, _p, _i64, _flt, _p, _i64, _p]),
     "jb_sgemm_bf16x3": (_int, [_ch, _ch, _i64, _i64, _i64, _flt, _p, _i64, _p, _i64, _flt, _p, _i64, _int]),
     "jb_sgemm_bf16x3_dev": (_int, [_ch, _ch, _i64, _i64, _i64, _flt, _p, _i64, _p, _i64, _flt, _p, _i64, _int, _p]),
+    "jb_nonfinite_check": (_int, [_p]),
     "jb_gemm_grouped_dev": (_int, [_int, _i64, _p, _p]),
     "jb_grouped_plan_create": (_int, [_int, _i64, _p, _p, C.POINTER(_p)]),
     "jb_grouped_plan_launch": (_int, [_p, _p]),
@@ -106,6 +107,9 @@ SIGNATURES = {
     "jb_fill_uniform_dev": (_int, [C.c_uint64, _i64, _p, _p]),
     "jb_fill_wide_dev": (_int, [C.c_uint64, _i64, _int, _int, _p, _p]),
     "jb_fill_uniform_f32_dev": (_int, [C.c_uint64, _i64, _p, _p]),
+    "jb_fill_wide_f32_dev": (_int, [C.c_uint64, _i64, _int, _int, _p, _p]),
+    "jb_fill_normal_dev": (_int, [C.c_uint64, _i64, _dbl, _p, _p]),
+    "jb_fill_normal_f32_dev": (_int, [C.c_uint64, _i64, _dbl, _p, _p]),
 }
 
 _lock = threading.Lock()

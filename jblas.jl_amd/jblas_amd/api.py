@@ -150,6 +150,13 @@ def gemm_(tA: str, tB: str, alpha, A, B, beta, C, *, method: str = "native", nsl
     return C
 
 
+def nonfinite_check(*, stream=None) -> None:
+    """Raises JBError (JB_ERR_NONFINITE) if an emulated call on the device
+    (split-BF16, split-INT8, 3xBF16) met Inf or NaN in an operand since the
+    previous check; synchronises `stream` (default: torch's current stream)."""
+    _lib.call("jb_nonfinite_check", _stream_of(True, stream))
+
+
 def jmul_(D, A, X, *, stream=None):
     """D = A*X (jmul!, src/gemm.jl:244-348): the reference's per-element fma
     recurrence, bit-identical, with the fringe rows/columns written too.
@@ -236,10 +243,33 @@ def fill_uniform_(out, seed: int, *, stream=None):
 
 
 def fill_wide_(out, seed: int, emin: int = -12, emax: int = 12, *, stream=None):
-    """Fill with u*2^e, u uniform in [-1,1), e uniform integer in [emin, emax]."""
-    if not (_is_torch(out) and out.is_cuda and out.is_contiguous() and out.dtype == torch.float64):
-        raise ValueError("out must be a contiguous float64 GPU tensor")
-    _lib.call("jb_fill_wide_dev", seed, out.numel(), emin, emax, out.data_ptr(), _stream_of(True, stream))
+    """Fill with u*2^e, u uniform in [-1,1), e uniform integer in [emin, emax]
+    (float32: the float64 value rounded to nearest)."""
+    if not (_is_torch(out) and out.is_cuda and out.is_contiguous()):
+        raise ValueError("out must be a contiguous GPU tensor")
+    s = _stream_of(True, stream)
+    if out.dtype == torch.float64:
+        _lib.call("jb_fill_wide_dev", seed, out.numel(), emin, emax, out.data_ptr(), s)
+    elif out.dtype == torch.float32:
+        _lib.call("jb_fill_wide_f32_dev", seed, out.numel(), emin, emax, out.data_ptr(), s)
+    else:
+        raise TypeError("float64 or float32 only")
+    return out
+
+
+def fill_normal_(out, seed: int, scale: float = 1.0, *, stream=None):
+    """Fill with scale * N(0, 1) draws (Box-Muller on the SplitMix64 stream; the
+    randn() of mrandn, src/randmat.jl:5-14).  Agrees with the host generator to
+    a few ulp (device log and cos)."""
+    if not (_is_torch(out) and out.is_cuda and out.is_contiguous()):
+        raise ValueError("out must be a contiguous GPU tensor")
+    s = _stream_of(True, stream)
+    if out.dtype == torch.float64:
+        _lib.call("jb_fill_normal_dev", seed, out.numel(), float(scale), out.data_ptr(), s)
+    elif out.dtype == torch.float32:
+        _lib.call("jb_fill_normal_f32_dev", seed, out.numel(), float(scale), out.data_ptr(), s)
+    else:
+        raise TypeError("float64 or float32 only")
     return out
 
 

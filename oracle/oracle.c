@@ -23,8 +23,10 @@
  *      fma, TwoSum of Knuth), with a double-double alpha/beta epilogue.
  *   5. orc_dgemm_ozaki : exact restatement of the split-BF16 emulated DGEMM
  *      (Ozaki, Ogita, Oishi & Rump, Numer. Algorithms 59, 2012, with
- *      exponent-scaled balanced 8-bit digits); integer level sums are formed
- *      exactly in int64, so the result bits are fully determined.
+ *      exponent-scaled balanced 9-bit digits, |d| <= 256, each exact in BF16);
+ *      integer level sums are formed exactly in int64, so the result bits are
+ *      fully determined.  orc_dgemm_ozaki_i8: the same with signed 8-bit
+ *      digits (the INT8-slice method).
  *   6. orc_pick_kernel_size / orc_blocking_structure : integer tile policy
  *      (src/kernel_structure.jl:76-99, src/memory_management.jl:78-177,
  *      src/miscellaneous.jl:8-14).
@@ -40,7 +42,8 @@
  *                      force on tiny inputs, numpy dgemm within gamma_K.
  *   orc_dgemm_ozaki -- inputs exactly representable in s digits give the
  *                      exact product; digits reconstruct the scaled input;
- *                      error vs Dot2 shrinks by ~2^-8 per added slice.
+ *                      error vs Dot2 shrinks by at least 2^-7 per added slice
+ *                      (9 bits per slice in the BF16 split, 8 in INT8).
  *   tile policy     -- hand-derived table for AVX2/AVX-512 register files.
  *
  * Build: oracle/Makefile (gcc, -ffp-contract=off so no fused operation is
@@ -765,4 +768,19 @@ ORC_API int orc_blocking_structure(int64_t M, int64_t N, int64_t P, int64_t t_si
   int64_t v[9] = {m1, n1, p1, m2, n2, p2, m3, n3, p3};
   memcpy(out, v, sizeof v);
   return 3;
+}
+
+/* Threads an OpenMP parallel region with num_threads(nthreads) actually runs
+ * (the CPU-baseline leg reports this count). */
+ORC_API int orc_threads_used(int nthreads) {
+  int used = 1;
+  (void)nthreads;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+  {
+#ifdef _OPENMP
+#pragma omp single
+    used = omp_get_num_threads();
+#endif
+  }
+  return used;
 }

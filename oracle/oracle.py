@@ -71,6 +71,8 @@ def lib():
                                      C.c_int]
     L.orc_ozaki_i8_chunk.argtypes = []
     L.orc_ozaki_i8_chunk.restype = C.c_int
+    L.orc_threads_used.argtypes = [C.c_int]
+    L.orc_threads_used.restype = C.c_int
     L.orc_kernel_acc_d.argtypes = [i64, i64, i64, p, i64, p, i64, p, i64]
     L.orc_kernel_acc_d.restype = None
     L.orc_pick_kernel_size.argtypes = [i64, i64, i64, p]
@@ -84,6 +86,11 @@ def lib():
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def threads_used(nthreads: int) -> int:
+    """Threads a parallel region of the restatement actually runs with."""
+    return int(lib().orc_threads_used(nthreads))
 
 
 def nthreads_default() -> int:

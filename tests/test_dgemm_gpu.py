@@ -193,7 +193,7 @@ def test_dgemm_c2_1024(gpu, ta, tb):
     assert scaled_err(got[ii, jj], d2, absab[ii, jj]) <= 2 * n * U
 
 
-@pytest.mark.parametrize("n,ta,tb", [(4096, "N", "N"), (4096, "T", "N"), (8192, "N", "N"), (8192, "N", "T")])
+@pytest.mark.parametrize("n,ta,tb", [(n, ta, tb) for n in (4096, 8192) for ta in "NT" for tb in "NT"])
 def test_dgemm_c2_large_sampled(gpu, n, ta, tb):
     """Config sizes 4096/8192: inputs generated in HBM (bit-identical to the host
     generator), sampled entries bit-checked against the recurrence and within
@@ -301,6 +301,25 @@ def test_beta_zero_never_reads_c_alpha_zero_k_zero(gpu):
     assert np.array_equal(C, -C0)
 
 
+@pytest.mark.parametrize("name,dtype,extra", [("jb_dgemm", np.float64, ()), ("jb_dgemm_recurrence", np.float64, ()),
+                                               ("jb_dgemm_ozaki", np.float64, (6,)),
+                                               ("jb_dgemm_ozaki_i8", np.float64, (7,)),
+                                               ("jb_sgemm", np.float32, ()), ("jb_sgemm_bf16x3", np.float32, (6,))])
+def test_host_entries_alpha_zero_null_operands(gpu, name, dtype, extra):
+    """alpha == 0 with m, n, k > 0: C = beta*C without reading A or B, so NULL
+    operands are accepted and nothing is staged from them."""
+    import ctypes
+
+    m, n, k = 37, 29, 41
+    C0 = G.matrix(6, m, n, dtype=dtype)
+    C = C0.copy(order="F")
+    L = jb.lib()
+    rc = getattr(L, name)(b"N", b"N", m, n, k, 0.0, None, m, None, k, 0.5, C.ctypes.data_as(ctypes.c_void_p), m,
+                          *extra)
+    assert rc == 0, jb.last_error()
+    assert np.array_equal(C, (dtype(0.5) * C0).astype(dtype))
+
+
 def test_fill_bit_identical(gpu):
     torch = gpu
     t = torch.empty(100003, dtype=torch.float64, device="cuda")
@@ -311,6 +330,26 @@ def test_fill_bit_identical(gpu):
     f = torch.empty(1001, dtype=torch.float32, device="cuda")
     jb.fill_uniform_(f, 5)
     assert np.array_equal(f.cpu().numpy(), G.uniform(5, 1001).astype(np.float32))
+    jb.fill_wide_(f, 6, -12, 12)
+    assert np.array_equal(f.cpu().numpy(), G.wide(6, 1001).astype(np.float32))
+
+
+def test_fill_normal(gpu):
+    """Device Box-Muller normals (mrandn's randn()) agree with the host
+    generator to rounding (device log and cos) and have N(0, 1) moments."""
+    torch = gpu
+    n = 1 << 20
+    t = torch.empty(n, dtype=torch.float64, device="cuda")
+    jb.fill_normal_(t, 77)
+    d, h = t.cpu().numpy(), G.normal(77, n)
+    assert np.max(np.abs(d - h) / np.maximum(1.0, np.abs(h))) <= 1e-14
+    assert abs(d.mean()) < 5 / np.sqrt(n) and abs(d.var() - 1.0) < 5 * np.sqrt(2.0 / n)
+    f = torch.empty(4097, dtype=torch.float32, device="cuda")
+    jb.fill_normal_(f, 78, 1.0 / np.sqrt(2048))
+    hf = (G.normal(78, 4097) / np.sqrt(2048)).astype(np.float32)
+    # f32: the same double rounded to float, or a neighbouring float where the
+    # device's double differs across a rounding boundary
+    assert np.max(np.abs(f.cpu().numpy().astype(np.float64) - hf) / np.maximum(2.0 ** -11, np.abs(hf))) <= 2.0 ** -23
 
 
 def test_sgemm_native_tolerance(gpu):

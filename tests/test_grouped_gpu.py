@@ -233,6 +233,26 @@ def test_grouped_plan_graph_replays(gpu):
     plan.destroy()
 
 
+def test_grouped_bf16x3_c4_config_set(gpu):
+    """Config: the 1024-problem set (m, n, k log-uniform in [16, 2048], seed 52)
+    in Float32 through the 3xBF16 split, one persistent launch; every 8th
+    problem within 2*K*2^-24 componentwise of the FP64 (Dot2) oracle."""
+    torch = gpu
+    probs, descs = make_group(torch, 1024, 52, torch.float32)
+    C0 = [to_host(p[7]) for p in probs[::8]]
+    jb.gemm_grouped_("f32_bf16x3", descs)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for idx, (ta, tb, m, n, k, A, B, C) in enumerate(probs[::8]):
+        Ah, Bh = to_host(A), to_host(B)
+        ref = O.sgemm_dot2(ta, tb, 1.0, Ah, Bh, 0.0, C0[idx])
+        absab = np.abs(Ah).astype(float) @ np.abs(Bh).astype(float)
+        err = np.max(np.abs(to_host(C).astype(float) - ref) / absab)
+        worst = max(worst, err / k)
+        assert err <= 2 * k * U32, (m, n, k, err)
+    print(f"max componentwise error / k over 128 problems: {worst:.3e} (bar 2^-23 = {2 * U32:.3e})")
+
+
 @pytest.mark.parametrize("method", ["f32", "f32_bf16x3"])
 def test_grouped_f32_tolerance(gpu, method):
     torch = gpu
@@ -311,6 +331,26 @@ def test_autotune_roundtrip(gpu):
         _, sk = jb.tune_lookup("f64", m, n, k)
         ref = O.gemm_recurrence_chunked("N", "N", 1.0, to_host(A), to_host(B), 0.0, C0[4 * idx],
                                         jb.split_chunk(k, sk))
+        assert np.array_equal(to_host(C), ref)
+    jb.tune_reset()
+
+
+def test_autotune_leaves_c_untouched(gpu):
+    """The tuner's trials write to scratch outputs: a beta != 0 group's C is the
+    same after tuning, and one launch afterwards computes alpha*A*B + beta*C once."""
+    torch = gpu
+    probs, descs = make_group(torch, 24, 57, torch.float64, alpha=1.5, beta=0.5)
+    C0 = [to_host(p[7]) for p in probs]
+    jb.tune_reset()
+    jb.autotune("f64", descs, reps=2)
+    torch.cuda.synchronize()
+    for p, c0 in zip(probs, C0):
+        assert np.array_equal(to_host(p[7]), c0)
+    jb.gemm_grouped_("f64", descs)
+    torch.cuda.synchronize()
+    for idx, (ta, tb, m, n, k, A, B, C) in enumerate(probs):
+        _, sk = jb.tune_lookup("f64", m, n, k)
+        ref = O.gemm_recurrence_chunked("N", "N", 1.5, to_host(A), to_host(B), 0.5, C0[idx], jb.split_chunk(k, sk))
         assert np.array_equal(to_host(C), ref)
     jb.tune_reset()
 

@@ -110,17 +110,6 @@ def test_ozaki_i8_degenerate_sizes_and_alpha_zero(gpu):
     assert ei.value.code == -14
 
 
-def test_ozaki_i8_nonfinite_is_an_error(gpu):
-    A = G.matrix(1, 40, 30)
-    B = G.matrix(2, 30, 20)
-    A[5, 7] = -np.inf
-    with pytest.raises(jb.JBError) as ei:
-        jb.gemm_("N", "N", 1.0, A, B, 0.0, np.zeros((40, 20), order="F"), method="ozaki_int8", nslices=4)
-    assert ei.value.code == 3
-    A[5, 7] = 1.0
-    C = np.zeros((40, 20), order="F")
-    jb.gemm_("N", "N", 1.0, A, B, 0.0, C, method="ozaki_int8", nslices=4)
-    assert np.array_equal(C, O.dgemm_ozaki_i8("N", "N", 1.0, A, B, 0.0, np.zeros((40, 20)), 4))
 
 
 def test_ozaki_i8_c3_8192_sampled(gpu):
