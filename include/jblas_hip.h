@@ -185,6 +185,28 @@ int jb_dgemm_ozaki_dev(char transA, char transB, int64_t m, int64_t n, int64_t k
                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                        double* C, int64_t ldc, int nslices, void* stream);
 
+/* Allocation-free form of jb_dgemm_ozaki_dev.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and bits as jb_dgemm_ozaki_dev; it calls no hipMalloc,
+ * hipFree or allocating routine.  Its workspace is a static device array of
+ * JB_NOALLOC_WORKSPACE_BYTES per GPU, held from the library's load until the
+ * process ends; the workspace of an m x n x k call with s slices is the sum of
+ *   2*s*Mp*Kp, 2*s*Np*Kp, 8*Mp, 8*Np, 4*Mp, 4*Np and 8 bytes, each term
+ *   rounded up to a multiple of 256 (Mp = m rounded up to 128, Np = n to 64,
+ *   Kp = k to 64), so m = n = k = 1600 fits at s = 6 (62.7 MB) and 1664 does
+ *   not.  A larger call returns JB_ERR_UNSUPPORTED
+ * before any HIP call.  The library must be initialised first (jb_init;
+ * JB_ERR_STATE otherwise).  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (two streams, or two threads whose streams
+ * are not ordered) race and give wrong results; calls ordered on a stream do
+ * not.  Non-finite operands: as jb_dgemm_ozaki_dev (NaN elements, reported by
+ * jb_nonfinite_check).  No per-kernel timing events are recorded. */
+#define JB_NOALLOC_WORKSPACE_BYTES ((size_t)64 << 20)
+int jb_dgemm_ozaki_noalloc_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
+                               const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                               int64_t ldc, int nslices, void* stream);
+
 /* Emulated FP64 on INT8 slices: 8-bit digit weights (2^(-7-8t) times the
  * row/column exponent) with each digit a signed byte of the operand rounded to
  * 8*nslices-1 bits below its row/column scale, slice pairs (t+u <= nslices+1)

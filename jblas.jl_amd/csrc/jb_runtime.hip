@@ -20,6 +20,18 @@
 
 namespace jb {
 
+// ---- scratch of the allocation-free emulated entry point -------------------
+// One array per device, allocated by the code-object loader and held until the
+// process ends.  Its first JB_SCRATCH_HEADER bytes hold the sticky non-finite
+// word of the allocation-free calls; the emulated workspace follows.  Host
+// code never uses the array's name as an address: each device's address comes
+// from hipGetSymbolAddress at init (g_scratch_ptr).
+constexpr size_t JB_SCRATCH_HEADER = 256;
+constexpr int JB_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[JB_SCRATCH_HEADER + JB_NOALLOC_WORKSPACE_BYTES];
+static void* g_scratch_ptr[JB_MAX_DEVICES];  // per device, set once under the runtime lock
+static thread_local bool tl_no_timing = false;  // allocation-free calls create no timing events
+
 Runtime& rt() {
   static Runtime r;
   return r;
@@ -93,6 +105,12 @@ int init_locked(int device) {
   if (e != hipSuccess || count == 0) return fail(JB_ERR_HIP, "no HIP device available");
   if (device < 0 || device >= count) return fail(-1, "device %d out of range [0,%d)", device, count);
   if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+  if (device < JB_MAX_DEVICES && g_scratch_ptr[device] == nullptr) {
+    void* sp = nullptr;
+    if ((e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch))) != hipSuccess)
+      return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = sp;
+  }
   hipDeviceProp_t p;
   if ((e = hipGetDeviceProperties(&p, device)) != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
   jb_device_info& d = r.info;
@@ -299,7 +317,7 @@ static std::map<std::string, std::pair<double, int64_t>> g_totals;
 int runtime_compute_units() { return rt().info.compute_units; }
 
 void timing_begin(hipStream_t s, const char* kernel) {
-  if (!g_timing || capturing(s)) return;
+  if (!g_timing || tl_no_timing || capturing(s)) return;
   TimingRec r;
   r.name = kernel;
   if (hipEventCreate(&r.a) != hipSuccess) return;
@@ -308,7 +326,7 @@ void timing_begin(hipStream_t s, const char* kernel) {
 }
 
 void timing_end(hipStream_t s, const char* kernel) {
-  if (!g_timing || capturing(s)) return;
+  if (!g_timing || tl_no_timing || capturing(s)) return;
   for (auto it = g_pending.rbegin(); it != g_pending.rend(); ++it) {
     if (it->b == nullptr && it->name == kernel) {
       if (hipEventCreate(&it->b) == hipSuccess) hipEventRecord(it->b, s);
@@ -639,6 +657,42 @@ int jb_dgemm_ozaki_dev(char transA, char transB, int64_t m, int64_t n, int64_t k
   return ozaki_dev_locked(g, nslices, false, (hipStream_t)stream);
 }
 
+// jb_dgemm_ozaki_dev without any allocation: the workspace is this device's
+// `scratch` array (address from init), the kernels are the same.
+int jb_dgemm_ozaki_noalloc_dev(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha,
+                               const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                               int64_t ldc, int nslices, void* stream) {
+  if (nslices < 1 || nslices > 8) return fail(-14, "nslices must be in 1..8 (got %d)", nslices);
+  GemmArgs g;
+  int rc = validate_gemm(transA, transB, m, n, k, alpha == 0, A, lda, B, ldb, C, ldc, &g);
+  if (rc) return rc;
+  g.alpha = alpha;
+  g.beta = beta;
+  const size_t need = ozaki_workspace_bytes(m, n, k, nslices);
+  if (need > JB_NOALLOC_WORKSPACE_BYTES)
+    return fail(JB_ERR_UNSUPPORTED,
+                "the allocation-free emulated DGEMM needs %zu bytes of workspace for %lld x %lld x %lld with %d "
+                "slices; its scratch holds %llu",
+                need, (long long)m, (long long)n, (long long)k, nslices,
+                (unsigned long long)JB_NOALLOC_WORKSPACE_BYTES);
+  std::lock_guard<std::mutex> lk(rt().mu);
+  Runtime& r = rt();
+  if (!r.inited) return fail(JB_ERR_STATE, "call jb_init first: the allocation-free entry point does not initialise");
+  if (m == 0 || n == 0) return JB_OK;
+  char* sp = r.device < JB_MAX_DEVICES ? static_cast<char*>(g_scratch_ptr[r.device]) : nullptr;
+  if (!sp) return fail(JB_ERR_STATE, "no scratch address for device %d", r.device);
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e;
+  if (alpha == 0.0 || k == 0) {
+    e = launch_scale_f64(m, n, beta, C, ldc, st);
+  } else {
+    tl_no_timing = true;
+    e = launch_dgemm_ozaki(g, nslices, sp + JB_SCRATCH_HEADER, reinterpret_cast<int*>(sp), st);
+    tl_no_timing = false;
+  }
+  return e == hipSuccess ? JB_OK : hip_fail(e, "dgemm_ozaki (allocation-free) launch");
+}
+
 int jb_dgemm_ozaki(char transA, char transB, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
                    int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int nslices) {
   if (nslices < 1 || nslices > 8) return fail(-14, "nslices must be in 1..8 (got %d)", nslices);
@@ -742,13 +796,22 @@ int jb_nonfinite_check(void* stream) {
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (capturing(s)) return fail(JB_ERR_STATE, "jb_nonfinite_check synchronises: not allowed during graph capture");
-  int h = 0;
-  hipError_t e = hipMemcpyAsync(&h, rt().sticky, sizeof(int), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e == hipSuccess && h) e = hipMemsetAsync(rt().sticky, 0, sizeof(int), s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) return hip_fail(e, "non-finite flag readback");
-  return h ? fail(JB_ERR_NONFINITE, "an emulated call since the last check met Inf or NaN in an operand") : JB_OK;
+  // the library's sticky word and the allocation-free calls' one (scratch header)
+  int* words[2] = {rt().sticky, nullptr};
+  const int dev = rt().device;
+  if (dev < JB_MAX_DEVICES && g_scratch_ptr[dev]) words[1] = static_cast<int*>(g_scratch_ptr[dev]);
+  int seen = 0;
+  for (int* w : words) {
+    if (!w) continue;
+    int h = 0;
+    hipError_t e = hipMemcpyAsync(&h, w, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess && h) e = hipMemsetAsync(w, 0, sizeof(int), s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "non-finite flag readback");
+    seen |= h;
+  }
+  return seen ? fail(JB_ERR_NONFINITE, "an emulated call since the last check met Inf or NaN in an operand") : JB_OK;
 }
 
 // ---- generator --------------------------------------------------------------

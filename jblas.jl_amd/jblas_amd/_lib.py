@@ -83,6 +83,8 @@ SIGNATURES = {
     "jb_dgemm_recurrence_dev": (_int, [_ch, _ch, _i64, _i64, _i64, _dbl, _p, _i64, _p, _i64, _dbl, _p, _i64, _p]),
     "jb_dgemm_ozaki": (_int, [_ch, _ch, _i64, _i64, _i64, _dbl, _p, _i64, _p, _i64, _dbl, _p, _i64, _int]),
     "jb_dgemm_ozaki_dev": (_int, [_ch, _ch, _i64, _i64, _i64, _dbl, _p, _i64, _p, _i64, _dbl, _p, _i64, _int, _p]),
+    "jb_dgemm_ozaki_noalloc_dev": (_int, [_ch, _ch, _i64, _i64, _i64, _dbl, _p, _i64, _p, _i64, _dbl, _p, _i64, _int,
+                                          _p]),
     "jb_dgemm_ozaki_i8": (_int, [_ch, _ch, _i64, _i64, _i64, _dbl, _p, _i64, _p, _i64, _dbl, _p, _i64, _int]),
     "jb_dgemm_ozaki_i8_dev": (_int, [_ch, _ch, _i64, _i64, _i64, _dbl, _p, _i64, _p, _i64, _dbl, _p, _i64, _int,
                                      _p]),
