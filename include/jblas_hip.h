@@ -108,6 +108,14 @@ int jb_jmul(int64_t M, int64_t N, int64_t P, double* D, const double* A, const d
 int jb_jmul_dev(int64_t M, int64_t N, int64_t P, double* D, const double* A, const double* X,
                 void* stream);
 
+/* jmul! in Float32 (the reference's jmul! is generic in T, src/gemm.jl:244-246):
+ * D = A * X with the same recurrence in binary32 -- acc_1 = fl(A[i,1]*X[1,j]),
+ * acc_n = fmaf(A[i,n], X[n,j], acc_{n-1}) in increasing n -- on a sequential-fma
+ * kernel (vector ALUs), bit-identical, fringe included.  Leading dimensions M,
+ * N, M.  Host pointers (synchronous) / device pointers on `stream`. */
+int jb_jmul_f32(int64_t M, int64_t N, int64_t P, float* D, const float* A, const float* X);
+int jb_jmul_f32_dev(int64_t M, int64_t N, int64_t P, float* D, const float* A, const float* X, void* stream);
+
 /* ---- kernel! / initkernel! drop-in ---------------------------------------------- */
 /* kernel!(pD, pA, pX, Kernel{M,P,stride_AD,stride_X,N}) src/kernels.jl:212-241:
  * D (M x P) += A (M x N) * X (N x P) with each element's chain starting from D:

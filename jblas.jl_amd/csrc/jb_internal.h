@@ -55,6 +55,8 @@ size_t emulated_flag_offset(int64_t m, int64_t n, int64_t k, int s, bool i8);
 
 // ---- FP32 -----------------------------------------------------------------
 hipError_t launch_sgemm_mfma(const GemmArgs& g, hipStream_t s);
+// Sequential-fma FP32 GEMM (the jmul! recurrence in Float32, bit-exact).
+hipError_t launch_sgemm_recurrence(const GemmArgs& g, hipStream_t s);
 size_t bf16x3_workspace_bytes(int64_t m, int64_t n, int64_t k);
 // Non-finite inputs become NaN slices (every product they enter is NaN) and set
 // the two flags as in launch_dgemm_ozaki.

@@ -493,6 +493,54 @@ int jb_jmul(int64_t M, int64_t N, int64_t P, double* D, const double* A, const d
   return jb_dgemm('N', 'N', M, P, N, 1.0, A, M > 1 ? M : 1, X, N, 0.0, D, M > 1 ? M : 1);
 }
 
+// jmul! in Float32 (the reference's jmul! is generic in T, src/gemm.jl:244):
+// the same recurrence in binary32 (plain first product, then fmaf in
+// increasing k) on the sequential-fma kernel, bit-identical.
+extern "C++" {
+static int jmul_f32_locked(int64_t M, int64_t N, int64_t P, float* D, const float* A, const float* X, void* stream,
+                           bool host) {
+  GemmArgs g{};
+  g.m = M;
+  g.n = P;
+  g.k = N;
+  g.alpha = 1.0;
+  g.beta = 0.0;
+  g.A = A;
+  g.lda = M > 1 ? M : 1;
+  g.B = X;
+  g.ldb = N;
+  g.C = D;
+  g.ldc = M > 1 ? M : 1;
+  auto run = [](const GemmArgs& d, hipStream_t s) {
+    const hipError_t e = launch_sgemm_recurrence(d, s);
+    return e == hipSuccess ? JB_OK : hip_fail(e, "sgemm_recurrence launch");
+  };
+  if (!host) return run(g, (hipStream_t)stream);
+  return run_staged(g, sizeof(float), false, run);
+}
+
+static int jmul_f32_entry(int64_t M, int64_t N, int64_t P, float* D, const float* A, const float* X, void* stream,
+                          bool host) {
+  int rc = jmul_validate(M, N, P, D, A, X);
+  if (rc) return rc;
+  if (M > (int64_t)1 << 30 || P > (int64_t)1 << 30) return fail(-1, "M or P above 2^30 is not supported");
+  if (((M + 63) / 64) * ((P + 63) / 64) >= ((int64_t)1 << 23))
+    return fail(JB_ERR_UNSUPPORTED, "D of %lld x %lld is too large for one launch", (long long)M, (long long)P);
+  std::lock_guard<std::mutex> lk(rt().mu);
+  if (M == 0 || P == 0) return JB_OK;
+  if ((rc = ensure_init())) return rc;
+  return jmul_f32_locked(M, N, P, D, A, X, stream, host);
+}
+}  // extern "C++"
+
+int jb_jmul_f32(int64_t M, int64_t N, int64_t P, float* D, const float* A, const float* X) {
+  return jmul_f32_entry(M, N, P, D, A, X, nullptr, true);
+}
+
+int jb_jmul_f32_dev(int64_t M, int64_t N, int64_t P, float* D, const float* A, const float* X, void* stream) {
+  return jmul_f32_entry(M, N, P, D, A, X, stream, false);
+}
+
 // ---- kernel! / initkernel! (strided microkernels) --------------------------
 static int kernel_validate(bool init, int64_t M, int64_t N, int64_t P, const void* D, int64_t ldd, const void* A,
                            int64_t lda, const void* X, int64_t ldx, GemmArgs* g) {

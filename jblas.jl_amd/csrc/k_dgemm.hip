@@ -347,20 +347,24 @@ __global__ __launch_bounds__(NT, 2) void gemm_mfma_kernel(int64_t M, int64_t N, 
 }
 
 // ---------------------------------------------------------------------------
-// (2) Sequential-fma FP64 GEMM (bit-exact recurrence): 64x64 block tile,
+// (2) Sequential-fma GEMM (bit-exact recurrence, FP64 or FP32): 64x64 block tile,
 //     each thread a 4x4 register tile, BK=16 LDS stage.
 // ---------------------------------------------------------------------------
 constexpr int RB = 64;
-constexpr int RSTRIDE = RB + 2;
+// LDS row stride: one 16-byte vector of padding (66 doubles, 68 floats), so rows
+// stay 16-byte aligned for the vector stores and start on different banks
+template <typename T>
+constexpr int rstride() { return RB + 16 / (int)sizeof(T); }
 
-template <bool TA, bool TB>
-__global__ __launch_bounds__(NT, 2) void dgemm_recurrence_kernel(int64_t M, int64_t N, int64_t K, double alpha,
-                                                                 const double* __restrict__ A, int64_t lda,
-                                                                 const double* __restrict__ B, int64_t ldb,
-                                                                 double beta, double* __restrict__ C,
-                                                                 int64_t ldc, int tiles_m, int tiles_n) {
-  __shared__ __attribute__((aligned(16))) double As[BK * RSTRIDE];
-  __shared__ __attribute__((aligned(16))) double Bs[BK * RSTRIDE];
+template <typename T, bool TA, bool TB>
+__global__ __launch_bounds__(NT, 2) void gemm_recurrence_kernel(int64_t M, int64_t N, int64_t K, T alpha,
+                                                                const T* __restrict__ A, int64_t lda,
+                                                                const T* __restrict__ B, int64_t ldb, T beta,
+                                                                T* __restrict__ C, int64_t ldc, int tiles_m,
+                                                                int tiles_n) {
+  constexpr int RSTRIDE = rstride<T>();
+  __shared__ __attribute__((aligned(16))) T As[BK * RSTRIDE];
+  __shared__ __attribute__((aligned(16))) T Bs[BK * RSTRIDE];
   const int nblocks = tiles_m * tiles_n;
   const int tile = xcd_remap(blockIdx.x, nblocks, 8);
   int tm, tn;
@@ -368,15 +372,14 @@ __global__ __launch_bounds__(NT, 2) void dgemm_recurrence_kernel(int64_t M, int6
   const int64_t m0 = (int64_t)tm * RB, n0 = (int64_t)tn * RB;
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
 
-  TileLoader<double, RB, !TA, false> la;
-  TileLoader<double, RB, TB, false> lb;
-  double acc[4][4];
+  TileLoader<T, RB, !TA, false> la;
+  TileLoader<T, RB, TB, false> lb;
+  T acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int64_t m = m0 + tx + 16 * i, n = n0 + ty + 16 * j;
-      acc[i][j] = 0.0;
+      acc[i][j] = T(0);
     }
 
   for (int64_t k0 = 0; k0 < K; k0 += BK) {
@@ -389,7 +392,7 @@ __global__ __launch_bounds__(NT, 2) void dgemm_recurrence_kernel(int64_t M, int6
     const int kmax = (int)min<int64_t>(BK, K - k0);
     int kk = 0;
     if (k0 == 0) {  // acc_1 = fl(a_1 * x_1)   (src/gemm.jl:86)
-      double a[4], b[4];
+      T a[4], b[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) a[i] = As[tx + 16 * i];
 #pragma unroll
@@ -401,7 +404,7 @@ __global__ __launch_bounds__(NT, 2) void dgemm_recurrence_kernel(int64_t M, int6
       kk = 1;
     }
     for (; kk < kmax; ++kk) {  // acc_n = fma(a_n, x_n, acc_{n-1})   (src/gemm.jl:165)
-      double a[4], b[4];
+      T a[4], b[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) a[i] = As[kk * RSTRIDE + tx + 16 * i];
 #pragma unroll
@@ -418,8 +421,8 @@ __global__ __launch_bounds__(NT, 2) void dgemm_recurrence_kernel(int64_t M, int6
     for (int j = 0; j < 4; ++j) {
       const int64_t m = m0 + tx + 16 * i, n = n0 + ty + 16 * j;
       if (m < M && n < N) {
-        double c = alpha * acc[i][j];
-        if (beta != 0.0) c = fma(beta, C[m + n * ldc], c);
+        T c = alpha * acc[i][j];
+        if (beta != T(0)) c = fma(beta, C[m + n * ldc], c);
         C[m + n * ldc] = c;
       }
     }
@@ -520,16 +523,18 @@ hipError_t launch_dgemm_mfma_acc(const GemmArgs& g, hipStream_t s) {
 }
 hipError_t launch_sgemm_mfma(const GemmArgs& g, hipStream_t s) { return launch_gemm_mfma<float>(g, s); }
 
-hipError_t launch_dgemm_recurrence(const GemmArgs& g, hipStream_t s) {
+template <typename T>
+hipError_t launch_gemm_recurrence(const GemmArgs& g, hipStream_t s, const char* name) {
   const int tiles_m = (int)((g.m + RB - 1) / RB), tiles_n = (int)((g.n + RB - 1) / RB);
   const int nblocks = tiles_m * tiles_n;
-  auto A = static_cast<const double*>(g.A);
-  auto B = static_cast<const double*>(g.B);
-  auto C = static_cast<double*>(g.C);
-#define JB_LAUNCH(TA, TB)                                                                                 \
-  hipLaunchKernelGGL((dgemm_recurrence_kernel<TA, TB>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, g.alpha, \
-                     A, g.lda, B, g.ldb, g.beta, C, g.ldc, tiles_m, tiles_n)
-  timing_begin(s, "dgemm_recurrence");
+  auto A = static_cast<const T*>(g.A);
+  auto B = static_cast<const T*>(g.B);
+  auto C = static_cast<T*>(g.C);
+  const T alpha = (T)g.alpha, beta = (T)g.beta;
+#define JB_LAUNCH(TA, TB)                                                                                    \
+  hipLaunchKernelGGL((gemm_recurrence_kernel<T, TA, TB>), dim3(nblocks), dim3(NT), 0, s, g.m, g.n, g.k, alpha, \
+                     A, g.lda, B, g.ldb, beta, C, g.ldc, tiles_m, tiles_n)
+  timing_begin(s, name);
   if (!g.ta && !g.tb) {
     JB_LAUNCH(false, false);
   } else if (!g.ta && g.tb) {
@@ -540,8 +545,16 @@ hipError_t launch_dgemm_recurrence(const GemmArgs& g, hipStream_t s) {
     JB_LAUNCH(true, true);
   }
 #undef JB_LAUNCH
-  timing_end(s, "dgemm_recurrence");
+  timing_end(s, name);
   return hipGetLastError();
+}
+
+hipError_t launch_dgemm_recurrence(const GemmArgs& g, hipStream_t s) {
+  return launch_gemm_recurrence<double>(g, s, "dgemm_recurrence");
+}
+
+hipError_t launch_sgemm_recurrence(const GemmArgs& g, hipStream_t s) {
+  return launch_gemm_recurrence<float>(g, s, "sgemm_recurrence");
 }
 
 

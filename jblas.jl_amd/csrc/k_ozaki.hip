@@ -21,13 +21,16 @@
 //      earlier ones, so a step needs two new fragments).  Operand stages stream
 //      into an LDS ring of 4 slots (s <= 6; 3 above, whose stages 4 would not
 //      fit) by direct global->LDS loads (global_load_lds_dwordx4), three (two)
-//      stages ahead, one piece per step, one s_barrier per stage.  Every kc_l k-values (kc_l*(l+1) <= 256 for the
+//      stages ahead (past the last stage the last one again, so every stage
+//      has the same loads in flight and no branch around them), one piece per
+//      step, one s_barrier per stage.  Every kc_l k-values (kc_l*(l+1) <= 256 for the
 //      l+1 pairs of level l, products |d_t d_u| <= 2^16, so a level sum stays
 //      within 2^24 and is exact whatever the MFMA's internal order) level l is
-//      flushed into an FP64 group accumulator right after its last MFMA of the
-//      stage, G_g = fma(S_l, 2^-9j, G_g) with l = 3g+j (exact: G_g is a
-//      multiple of 2^-18 below 2^35 while k*s <= 2^18; for s > 6, l = 4g+j),
-//      so the flush overlaps the remaining MFMAs.  The epilogue combines the groups by
+//      flushed into an FP64 group accumulator after the stage's last MFMA
+//      (behind one branch on the stage's chunk-end mask; the two-stage levels
+//      flush branch-free in the next stage), G_g = fma(S_l, 2^-9j, G_g) with
+//      l = 3g+j (exact: G_g is a multiple of 2^-18 below 2^35 while
+//      k*s <= 2^18; for s > 6, l = 4g+j).  The epilogue combines the groups by
 //      TwoSum from the smallest, rescales by 2^(E_i+F_j-16) and applies
 //      alpha/beta.  These are the oracle restatement's operations in the same
 //      order, so the result is bit-reproducible.
@@ -53,6 +56,7 @@ typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 typedef int i4v __attribute__((ext_vector_type(4)));
 typedef int i16v __attribute__((ext_vector_type(16)));
 #define JB_LDS __attribute__((address_space(3)))
+#define JB_INLINE __attribute__((always_inline))  // lambdas of the kernels: never outlined
 
 constexpr int QBM = 128, QBN = 64;  // block tile (4/WB x 2 waves of 32*WB x 32)
 constexpr int QBK = 16;             // k per LDS stage = one MFMA k-step
@@ -494,7 +498,7 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   const uint16_t* gB = p.pb + (int64_t)tn * QBN * p.Kpad;
   // Piece i of this wave's share of stage ks (pieces j = wave + NWAVES*i; A:
   // plane t, 4 pieces of 1 KB; B: plane t, 2 pieces) -> LDS slot `slot`.
-  auto issue_piece = [&](int ks, int slot, int i) {
+  auto issue_piece = [&](int ks, int slot, int i) JB_INLINE {
     const int j = wave + NWAVES * i;  // uniform
     if (j >= NWL) return;
     const uint16_t* src;
@@ -511,7 +515,7 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
     __builtin_amdgcn_global_load_lds((const void*)(src + (uint32_t)dma_off), (JB_LDS void*)(ring + slot * STAGE_B + loff),
                                      16, 0, 0);
   };
-  auto issue = [&](int ks, int slot) {
+  auto issue = [&](int ks, int slot) JB_INLINE {
 #pragma unroll
     for (int i = 0; i < WL_MAX; ++i) issue_piece(ks, slot, i);
   };
@@ -543,7 +547,7 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
 
   // Flush of level j: G_g += S_j * 2^(-DB*(j%GL)) (S_j an exact integer, |S_j| <= 2^24);
   // the update itself is exact while G_g fits 53 bits (groups of three, k*s <= 2^18).
-  auto flush = [&](const int j, const bool zero) {
+  auto flush = [&](const int j, const bool zero) JB_INLINE {
     const double wj = __builtin_ldexp(1.0, -DB * (j % GL));  // folds to a constant
 #pragma unroll
     for (int b = 0; b < WB; ++b)
@@ -562,101 +566,108 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   auto fast = [&](const int L) { return MODE == 0 && S <= 6 && kc_stages(L) == 2; };  // (S > 6 would spill)
 
   const int nst = (int)(p.Kpad / QBK);
-  if constexpr (PD > 2) {
+  // The prefetch is unconditional: past the last stage it loads the last stage
+  // again, into the slot of a stage every wave is done with, so every stage has
+  // the same loads in flight and the stage code has no branch around its loads
+  // (branches split the MFMA stream into basic blocks the scheduler cannot
+  // overlap).  The kernel waits for those loads before it ends.
 #pragma unroll
-    for (int i = 0; i < PD; ++i)
-      if (i < nst) issue(i, i);
-  } else {
-    issue(0, 0);
-    if (nst > 1) issue(1, 1);
-  }
-  auto stage = [&](const int st, auto phase) {
+  for (int i = 0; i < PD; ++i) issue(min(i, nst - 1), i);
+  // chunk-end mask of the non-fast levels after stage st (bit l: level l's exact
+  // chunk ends here and it is flushed), the stage loop's runtime form
+  auto chunk_mask = [&](const int st) JB_INLINE {
+    int m = 0;
+#pragma unroll
+    for (int l = 0; l < NL; ++l)
+      if (MODE != 1 && !fast(l) && (st + 1 == nst || ((st + 1) & (kc_stages(l) - 1)) == 0)) m |= 1 << l;
+    return m;
+  };
+  auto stage = [&](const int st, auto phase, auto cmask) JB_INLINE {
     constexpr int PH = decltype(phase)::value;
-    // this stage landed; the later stages' loads (this wave's count each) may be in flight
-    if constexpr (PD > 2) {
-      const int ahead = min(PD - 1, nst - 1 - st);  // stages issued beyond st, still in flight
-      if (ahead >= 2) {
-        if (wave < NWL % NWAVES) wait_vmcnt<2 * WL_MAX>();
-        else wait_vmcnt<2 * WL_MIN>();
-      } else if (ahead == 1) {
-        if (wave < NWL % NWAVES) wait_vmcnt<WL_MAX>();
-        else wait_vmcnt<WL_MIN>();
-      }
-      else wait_vmcnt<0>();
-    } else {
-      if (st + 1 < nst) {
-        if (wave < NWL % NWAVES) wait_vmcnt<WL_MAX>();
-        else wait_vmcnt<WL_MIN>();
-      }
-      else wait_vmcnt<0>();
-    }
+    const int mask = cmask;  // a constant where the caller knows the stage index statically
+    // this stage landed; the PD - 1 later stages' loads (this wave's count each) may be in flight
+    if (wave < NWL % NWAVES) wait_vmcnt<(PD - 1) * WL_MAX>();
+    else wait_vmcnt<(PD - 1) * WL_MIN>();
     block_barrier();  // every wave's loads of stage st landed, every wave is done with st-1
     // Stage st+PD goes into the slot of st-1, one piece per step below: a burst of
     // loads right after the barrier fills the memory pipeline's queue and holds
     // up the wave's MFMA issue behind it.
-    const bool prefetch = st + PD < nst;
+    const int pstage = min(st + PD, nst - 1);
     const int pslot = (st + PD) % NSLOT;
     const u4* L = reinterpret_cast<const u4*>(ring + (st % NSLOT) * STAGE_B);
-    const int st1 = st + 1;  // a level's chunk ends after this stage if st1 is a multiple of its length
-    const bool last_stage = st1 == nst;
     // Pairs run in square order: step j multiplies the new slices A_j and B_j
     // with all earlier ones (pairs with max(t, u) == j).  A step needs only two
     // new fragments, read one step ahead, so a stage starts after two LDS reads
     // instead of all of them.  Level l = t + u is complete after step l; its
     // first pair is (t, u) = ((l+1)/2, l-(l+1)/2), at step (l+1)/2.
-    bf8 af[S][WB], bf[S];
-#pragma unroll
-    for (int b = 0; b < WB; ++b) af[0][b] = __builtin_bit_cast(bf8, L[(wm0 + 32 * b + l32) * 2 + phl]);
-    bf[0] = __builtin_bit_cast(bf8, L[(QBM + wn0 + l32) * 2 + phl]);
     constexpr int PPS = (WL_MAX + S - 1) / S;  // load pieces per step
+    auto mfma = [&](const int t, const int u, const bf8& b_frag, const bf8 (&a_frag)[WB], const bool first_of_level) JB_INLINE {
+      if (t + u + 2 > LMAX) return;
+      const int l = (MODE != 1) ? (t + u) : (t + u == 0 ? 0 : 1);
+      // first MFMA of a fast level in a phase-0 stage: flush the chunk that ended
+      // with the previous stage (zeros at st == 0), then start from zero
+      const bool restart = PH == 0 && fast(l) && first_of_level;
+      if (restart) flush(l, false);
+      // X = op(B) plane (p -> n), Y = op(A) plane (q -> m): item r of lane l holds
+      // C[m0+wm0+32b+l%32][n0+wn0+8(r/4)+4(l/32)+r%4].
 #pragma unroll
-    for (int j = 0; j < S; ++j) {
-      if (prefetch) {
+      for (int b = 0; b < WB; ++b)
+        if constexpr (MODE == 2)
+          acc[l][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(__builtin_bit_cast(i4v, b_frag),
+                                                            __builtin_bit_cast(i4v, a_frag[b]), acc[l][b], 0, 0, 0);
+        else
+          acc[l][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b_frag, a_frag[b], restart ? AccV{} : acc[l][b], 0, 0, 0);
+    };
+    auto frag_a = [&](const int t, const int b) JB_INLINE {
+      return __builtin_bit_cast(bf8, L[t * (PLANE_B / 16) + (wm0 + 32 * b + l32) * 2 + phl]);
+    };
+    auto frag_b = [&](const int u) JB_INLINE { return __builtin_bit_cast(bf8, L[u * (PLANE_B / 16) + (QBM + wn0 + l32) * 2 + phl]); };
+    {
+      bf8 af[S][WB], bf[S];
 #pragma unroll
-        for (int i = PPS * j; i < PPS * (j + 1) && i < WL_MAX; ++i) issue_piece(st + PD, pslot, i);
+      for (int b = 0; b < WB; ++b) af[0][b] = frag_a(0, b);
+      bf[0] = frag_b(0);
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+#pragma unroll
+        for (int i = PPS * j; i < PPS * (j + 1) && i < WL_MAX; ++i) issue_piece(pstage, pslot, i);
+        if (j + 1 < S) {
+#pragma unroll
+          for (int b = 0; b < WB; ++b) af[j + 1][b] = frag_a(j + 1, b);
+          bf[j + 1] = frag_b(j + 1);
+        }
+#pragma unroll
+        for (int q = 0; q < 2 * j + 1; ++q) {
+          const int t = q <= j ? j : q - j - 1;  // (j, 0..j), then (0..j-1, j)
+          const int u = q <= j ? q : j;
+          const int l = t + u;
+          mfma(t, u, bf[u], af[t], t == (l + 1) / 2 && u == l - (l + 1) / 2);
+        }
       }
-      if (j + 1 < S) {
+    }
+    // The levels whose exact chunk ends with this stage, in increasing level order,
+    // after the stage's last MFMA (their own last MFMAs were issued steps earlier):
+    // one branch on the mask, taken at chunk ends only.
+    if (MODE != 1 && mask != 0) {
 #pragma unroll
-        for (int b = 0; b < WB; ++b)
-          af[j + 1][b] = __builtin_bit_cast(bf8, L[(j + 1) * (PLANE_B / 16) + (wm0 + 32 * b + l32) * 2 + phl]);
-        bf[j + 1] = __builtin_bit_cast(bf8, L[(j + 1) * (PLANE_B / 16) + (QBM + wn0 + l32) * 2 + phl]);
-      }
-#pragma unroll
-      for (int q = 0; q < 2 * j + 1; ++q) {
-        const int t = q <= j ? j : q - j - 1;  // (j, 0..j), then (0..j-1, j)
-        const int u = q <= j ? q : j;
-        if (t + u + 2 > LMAX) continue;
-        const int l = (MODE != 1) ? (t + u) : (t + u == 0 ? 0 : 1);
-        // first MFMA of a fast level in a phase-0 stage: flush the chunk that ended
-        // with the previous stage (zeros at st == 0), then start from zero
-        const bool restart = PH == 0 && fast(l) && t == (l + 1) / 2 && u == l - (l + 1) / 2;
-        if (restart) flush(l, false);
-        // X = op(B) plane (p -> n), Y = op(A) plane (q -> m): item r of lane l holds
-        // C[m0+wm0+32b+l%32][n0+wn0+8(r/4)+4(l/32)+r%4].
-#pragma unroll
-        for (int b = 0; b < WB; ++b)
-          if constexpr (MODE == 2)
-            acc[l][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(__builtin_bit_cast(i4v, bf[u]),
-                                                              __builtin_bit_cast(i4v, af[t][b]), acc[l][b], 0, 0, 0);
-          else
-            acc[l][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bf[u], af[t][b], restart ? AccV{} : acc[l][b], 0, 0, 0);
-      }
-      if (MODE != 1 && j < NL && !fast(j) && (last_stage || (st1 & (kc_stages(j) - 1)) == 0))
-        flush(j, true);  // level j has received its last product of this chunk
+      for (int l = 0; l < NL; ++l)
+        if ((mask >> l) & 1) flush(l, true);
     }
   };
   if constexpr (MODE == 0) {
+    int st = 0;
     // nst is a multiple of 2 (the planes' k padding is a multiple of 64); the
     // guard keeps an odd count correct too (the last chunk then ends in phase 0)
-    for (int st = 0; st < nst; st += 2) {
-      stage(st, std::integral_constant<int, 0>());
-      if (st + 1 < nst) stage(st + 1, std::integral_constant<int, 1>());
+    for (; st < nst; st += 2) {
+      stage(st, std::integral_constant<int, 0>(), chunk_mask(st));
+      if (st + 1 < nst) stage(st + 1, std::integral_constant<int, 1>(), chunk_mask(st + 1));
     }
   } else {
     // no fast levels: one stage per iteration (pairing the stages cost the
     // 3xBF16 kernel ~20 %)
-    for (int st = 0; st < nst; ++st) stage(st, std::integral_constant<int, 1>());
+    for (int st = 0; st < nst; ++st) stage(st, std::integral_constant<int, 1>(), chunk_mask(st));
   }
+  wait_vmcnt<0>();  // the last stages' repeated prefetch lands before the workgroup's LDS is released
   if (MODE == 0) {
 #pragma unroll
     for (int j = 0; j < NL; ++j)

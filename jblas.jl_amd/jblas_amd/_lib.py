@@ -69,6 +69,8 @@ SIGNATURES = {
     "jb_kernel_timing_reset": (_int, []),
     "jb_jmul": (_int, [_i64, _i64, _i64, _p, _p, _p]),
     "jb_jmul_dev": (_int, [_i64, _i64, _i64, _p, _p, _p, _p]),
+    "jb_jmul_f32": (_int, [_i64, _i64, _i64, _p, _p, _p]),
+    "jb_jmul_f32_dev": (_int, [_i64, _i64, _i64, _p, _p, _p, _p]),
     "jb_kernel": (_int, [_i64, _i64, _i64, _p, _i64, _p, _i64, _p, _i64]),
     "jb_kernel_dev": (_int, [_i64, _i64, _i64, _p, _i64, _p, _i64, _p, _i64, _p]),
     "jb_initkernel": (_int, [_i64, _i64, _i64, _p, _i64, _p, _i64, _p, _i64]),

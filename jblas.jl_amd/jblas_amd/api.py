@@ -160,21 +160,27 @@ def nonfinite_check(*, stream=None) -> None:
 def jmul_(D, A, X, *, stream=None):
     """D = A*X (jmul!, src/gemm.jl:244-348): the reference's per-element fma
     recurrence, bit-identical, with the fringe rows/columns written too.
-    Float64 only, like the reference's tested path."""
+    Float64 (FP64 MFMA kernel, whose chains are that recurrence) or Float32
+    (the recurrence in binary32 on the sequential-fma kernel; the reference's
+    jmul! is generic in T, dense operands only)."""
     d, a, x = _Operand(D, True, "D"), _Operand(A, False, "A"), _Operand(X, False, "X")
     M, N = a.rows, a.cols
     if x.rows != N or d.rows != M or d.cols != x.cols:
         raise DimensionMismatch(f"D is {d.rows}x{d.cols}, A is {M}x{N}, X is {x.rows}x{x.cols}")
-    if not (d.dtype == a.dtype == x.dtype == np.float64):
-        raise TypeError("jmul! drop-in takes Float64 matrices")
+    if not (d.dtype == a.dtype == x.dtype and d.dtype in (np.float64, np.float32)):
+        raise TypeError("jmul! drop-in takes Float64 or Float32 matrices of one type")
     P = x.cols
-    if d.ld != max(1, M) or a.ld != max(1, M) or x.ld != max(1, N):
+    dense = d.ld == max(1, M) and a.ld == max(1, M) and x.ld == max(1, N)
+    suffix = "" if d.dtype == np.float64 else "_f32"
+    if not dense:
+        if suffix:
+            raise ValueError("Float32 jmul! takes dense column-major matrices (leading dimensions M, N, M)")
         # Strided views go through the BLAS form with explicit leading dimensions.
         return gemm_("N", "N", 1.0, A, X, 0.0, D, method="recurrence", stream=stream)
     if d.device:
-        _lib.call("jb_jmul_dev", M, N, P, d.ptr, a.ptr, x.ptr, _stream_of(True, stream))
+        _lib.call("jb_jmul" + suffix + "_dev", M, N, P, d.ptr, a.ptr, x.ptr, _stream_of(True, stream))
     else:
-        _lib.call("jb_jmul", M, N, P, d.ptr, a.ptr, x.ptr)
+        _lib.call("jb_jmul" + suffix, M, N, P, d.ptr, a.ptr, x.ptr)
     return D
 
 

@@ -90,6 +90,34 @@ def test_jmul_reference_shapes(gpu, M, N, P):
     assert np.array_equal(D, O.jmul(A, X))
 
 
+def test_jmul_f32_c1_and_reference_shapes(gpu):
+    """jmul! in Float32 (the reference's jmul! is generic in T): the binary32
+    recurrence, bit-exact against the oracle's Float32 restatement at 256^3
+    (host and device pointers) and at the reference's shapes plus fringes."""
+    A = G.matrix(42, 256, 256, dtype=np.float32)
+    B = G.matrix(43, 256, 256, dtype=np.float32)
+    ref = O.jmul(A, B)
+    assert ref.dtype == np.float32
+    D = np.zeros((256, 256), dtype=np.float32, order="F")
+    jb.jmul_(D, A, B)
+    assert np.array_equal(D.view(np.int32), ref.view(np.int32))
+    Dd = to_dev(gpu, np.zeros((256, 256), dtype=np.float32))
+    jb.jmul_(Dd, to_dev(gpu, A), to_dev(gpu, B))
+    gpu.cuda.synchronize()
+    assert np.array_equal(to_host(Dd).view(np.int32), ref.view(np.int32))
+    for (M, N, P) in [(16, 32, 14), (32, 32, 28), (128, 128, 126), (13, 12, 12), (800, 900, 840), (1, 1, 1),
+                      (3, 1000, 5), (65, 17, 130)]:
+        A = G.matrix(1, M, N, dtype=np.float32)
+        X = G.matrix(2, N, P, dtype=np.float32)
+        D = np.full((M, P), np.nan, dtype=np.float32, order="F")
+        jb.jmul_(D, A, X)
+        assert np.array_equal(D.view(np.int32), O.jmul(A, X).view(np.int32)), (M, N, P)
+    # signed zeros: the first step is a plain product, so -1 * 0 stays -0
+    D = np.ones((1, 1), dtype=np.float32, order="F")
+    jb.jmul_(D, np.array([[-1.0]], dtype=np.float32), np.array([[0.0]], dtype=np.float32))
+    assert np.signbit(D[0, 0]) and D[0, 0] == 0
+
+
 def test_jmul_sign_of_zero(gpu):
     D = np.ones((1, 1), order="F")
     jb.jmul_(D, np.array([[-1.0]]), np.array([[0.0]]))
