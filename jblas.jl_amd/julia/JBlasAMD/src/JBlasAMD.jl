@@ -20,14 +20,22 @@ jBLAS.jl's GEMM entry point.  Drop-ins:
   * `GroupedPlan` -- many GEMMs in one persistent, hipGraph-captured launch.
 
 All calls go through `ccall` into the C ABI declared in include/jblas_hip.h.
-Host arrays use the synchronous host-pointer entry points; device arrays
-(e.g. AMDGPU.ROCArray) can be passed to the `*_dev` functions as raw pointers.
+Host arrays use the synchronous host-pointer entry points.  Package
+extensions add the reference's own signatures and device arrays:
+  * StaticArrays: `jmul!(D::MMatrix, A::MMatrix, X::MMatrix, ::Val...)` and
+    `fastmul!(::MMatrix...)` on `pointer()` (src/gemm.jl:244-246,
+    src/kernels.jl:202-208);
+  * jBLAS: methods of `jBLAS.jmul!`, `jBLAS.fastmul!`, `jBLAS.kernel!` and
+    `jBLAS.initkernel!` for Float64 (and Float32 `jmul!`), so that an unchanged
+    jBLAS call runs here;
+  * AMDGPU: `jmul!`, `gemm!`, `fastmul!`, `kernel!`, `initkernel!` on
+    `ROCMatrix` through the `*_dev` entry points, on the array's HIP stream.
 """
 module JBlasAMD
 
 using LinearAlgebra
 
-export jmul!, fastmul!, gemm!, gemm, kernel!, initkernel!, GroupedPlan, launch!, destroy!
+export jmul!, fastmul!, gemm!, gemm, kernel!, initkernel!, Kernel, GroupedPlan, launch!, destroy!, nonfinite_check
 
 const libjblas = get(ENV, "JB_LIBRARY", normpath(joinpath(@__DIR__, "..", "..", "..", "lib", "libjblas_hip.so")))
 
@@ -50,7 +58,11 @@ check(func::Symbol, rc::Cint) = rc == 0 ? nothing : throw(JBError(func, rc, last
 colmajor(A::StridedMatrix) = stride(A, 1) == 1
 ld(A::StridedMatrix) = max(1, stride(A, 2), size(A, 1))
 
-function jmul!(D::StridedMatrix{Float64}, A::StridedMatrix{Float64}, X::StridedMatrix{Float64})
+# The reference's jmul! takes `Val` prefetch knobs after the operands
+# (src/gemm.jl:244-246); they tune CPU software prefetching and have no
+# counterpart on the GPU, so they are accepted and ignored.
+function jmul!(D::StridedMatrix{Float64}, A::StridedMatrix{Float64}, X::StridedMatrix{Float64},
+               ::Val = Val(7), ::Val = Val(7), ::Val = Val(3), ::Val = Val(3), ::Val = Val(3))
     M, N = size(A)
     N2, P = size(X)
     (N == N2 && size(D) == (M, P)) || throw(DimensionMismatch("D $(size(D)), A $(size(A)), X $(size(X))"))
@@ -61,6 +73,35 @@ function jmul!(D::StridedMatrix{Float64}, A::StridedMatrix{Float64}, X::StridedM
     end
     gemm!('N', 'N', 1.0, A, X, 0.0, D; method = :recurrence)
 end
+
+"Float32 jmul! (the reference's is generic in T): the binary32 recurrence, dense matrices."
+function jmul!(D::StridedMatrix{Float32}, A::StridedMatrix{Float32}, X::StridedMatrix{Float32},
+               ::Val = Val(7), ::Val = Val(7), ::Val = Val(3), ::Val = Val(3), ::Val = Val(3))
+    M, N, P = _fastmul_dims(D, A, X)
+    check(:jb_jmul_f32, ccall((:jb_jmul_f32, libjblas), Cint,
+                              (Int64, Int64, Int64, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}), M, N, P, D, A, X))
+    D
+end
+
+"""
+    jmul_ptr!(pD, pA, pX, M, N, P)
+
+jmul! on raw host pointers to dense column-major storage (leading dimensions
+M, N, M), the form the reference's own generated code works on (`pointer(D)`,
+src/gemm.jl:309-311).  Used by the StaticArrays extension for `MMatrix`.
+"""
+jmul_ptr!(pD::Ptr{Float64}, pA::Ptr{Float64}, pX::Ptr{Float64}, M::Integer, N::Integer, P::Integer) =
+    check(:jb_jmul, ccall((:jb_jmul, libjblas), Cint,
+                          (Int64, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}), M, N, P, pD, pA, pX))
+jmul_ptr!(pD::Ptr{Float32}, pA::Ptr{Float32}, pX::Ptr{Float32}, M::Integer, N::Integer, P::Integer) =
+    check(:jb_jmul_f32, ccall((:jb_jmul_f32, libjblas), Cint,
+                              (Int64, Int64, Int64, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}), M, N, P, pD, pA, pX))
+fastmul_ptr!(pD::Ptr{Float64}, pA::Ptr{Float64}, pX::Ptr{Float64}, M::Integer, N::Integer, P::Integer) =
+    check(:jb_fastmul, ccall((:jb_fastmul, libjblas), Cint,
+                             (Int64, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}), M, N, P, pD, pA, pX))
+fastmul_ptr!(pD::Ptr{Float32}, pA::Ptr{Float32}, pX::Ptr{Float32}, M::Integer, N::Integer, P::Integer) =
+    check(:jb_fastmul_f32, ccall((:jb_fastmul_f32, libjblas), Cint,
+                                 (Int64, Int64, Int64, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}), M, N, P, pD, pA, pX))
 
 function _fastmul_dims(D, A, X)
     M, N = size(A)
@@ -163,6 +204,59 @@ function initkernel!(D::StridedMatrix{Float64}, A::StridedMatrix{Float64}, X::St
         M, N, P, D, ld(D), A, ld(A), X, ld(X)))
     D
 end
+
+"""
+    Kernel{Mₖ,Pₖ,stride_AD,stride_X,N}
+    Kernel(Mₖ, N, Pₖ, stride_AD, stride_X)
+
+The reference's type-level microkernel descriptor (src/kernel_structure.jl:8-9):
+an Mₖ×Pₖ tile of D = A·X with K = N, A and D sharing the leading dimension
+stride_AD, X with leading dimension stride_X.  Same parameters, same
+constructor argument order.
+"""
+struct Kernel{Mₖ,Pₖ,stride_AD,stride_X,N} end
+Kernel(Mₖ, N, Pₖ, stride_AD, stride_X) = Kernel{Mₖ,Pₖ,stride_AD,stride_X,N}()
+
+"""
+    kernel!(pD::Ptr{Float64}, pA, pX, ::Kernel{Mₖ,Pₖ,stride_AD,stride_X,N}[, prefetch])
+
+The reference's pointer microkernel (src/kernels.jl:239-241, prefetch variants
+:277-537): D += A·X on the Mₖ×Pₖ tile at pD, each element's fma chain starting
+from D, with ldd = lda = stride_AD and ldx = stride_X.  Host pointers; the
+prefetch argument (a CPU software-prefetch schedule) is accepted and ignored.
+Returns `nothing`, as the reference.
+"""
+function kernel!(pD::Ptr{Float64}, pA::Ptr{Float64}, pX::Ptr{Float64}, ::Kernel{Mₖ,Pₖ,stride_AD,stride_X,N},
+                 pf = nothing) where {Mₖ,Pₖ,stride_AD,stride_X,N}
+    check(:jb_kernel, ccall((:jb_kernel, libjblas), Cint,
+        (Int64, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64),
+        Mₖ, N, Pₖ, pD, stride_AD, pA, stride_AD, pX, stride_X))
+    nothing
+end
+
+"""
+    initkernel!(pD::Ptr{Float64}, pA, pX, ::Kernel{Mₖ,Pₖ,stride_AD,stride_X,N}[, prefetch])
+
+The reference's initialising microkernel (src/kernels.jl:273-275): D = A·X on
+the tile, the first k-step a plain product, then fma (the jmul! recurrence).
+"""
+function initkernel!(pD::Ptr{Float64}, pA::Ptr{Float64}, pX::Ptr{Float64}, ::Kernel{Mₖ,Pₖ,stride_AD,stride_X,N},
+                     pf = nothing) where {Mₖ,Pₖ,stride_AD,stride_X,N}
+    check(:jb_initkernel, ccall((:jb_initkernel, libjblas), Cint,
+        (Int64, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64),
+        Mₖ, N, Pₖ, pD, stride_AD, pA, stride_AD, pX, stride_X))
+    nothing
+end
+
+"""
+    nonfinite_check(stream = C_NULL)
+
+Throws `JBError` (code 3) if an emulated call on device pointers (split-BF16,
+split-INT8, 3xBF16) met Inf or NaN in an operand since the previous check
+(those calls do not synchronise; the C elements such operands feed are NaN).
+"""
+nonfinite_check(stream::Ptr{Cvoid} = C_NULL) =
+    check(:jb_nonfinite_check, ccall((:jb_nonfinite_check, libjblas), Cint, (Ptr{Cvoid},), stream))
 
 "Non-mutating α*op(A)*op(B); differentiable through the ChainRulesCore extension."
 function gemm(tA::AbstractChar, tB::AbstractChar, α::Real, A::StridedMatrix{T}, B::StridedMatrix{T};

@@ -31,13 +31,22 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.SIGNATURES) == syms
 
 
+JL = os.path.join(ROOT, "jblas.jl_amd", "julia", "JBlasAMD")
+
+
+def _julia_sources():
+    files = [os.path.join(JL, "src", "JBlasAMD.jl")]
+    files += sorted(os.path.join(JL, "ext", f) for f in os.listdir(os.path.join(JL, "ext")) if f.endswith(".jl"))
+    return files
+
+
 def test_julia_binding_names_declared_symbols():
-    """The Julia wrapper (shipped as source; no Julia in this image) only
-    ccalls symbols the header declares, with the header's argument count."""
-    src = os.path.join(ROOT, "jblas.jl_amd", "julia", "JBlasAMD", "src", "JBlasAMD.jl")
-    text = open(src).read()
+    """The Julia wrapper and its extensions (shipped as source; no Julia in this
+    image) only ccall symbols the header declares, with the header's argument
+    count."""
+    text = "\n".join(open(f).read() for f in _julia_sources())
     called = re.findall(r"ccall\(\(:(jb_[a-z0-9_]+), libjblas\), \w+,\s*\(([^)]*)\)", text)
-    assert len(called) >= 10
+    assert len(called) >= 30
     header = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
     for name, argtypes in called:
         m = re.search(r"\b" + name + r"\s*\(([^)]*)\)", header)
@@ -46,6 +55,34 @@ def test_julia_binding_names_declared_symbols():
         argtypes = argtypes.strip().rstrip(",")  # Julia 1-tuples carry a trailing comma
         n_julia = 0 if not argtypes else argtypes.count(",") + 1
         assert n_header == n_julia, (name, n_header, n_julia)
+
+
+def test_julia_reference_signatures():
+    """The reference's own call forms have methods: jmul!(D::MMatrix{M,P,T},
+    A::MMatrix{M,N,T}, X::MMatrix{N,P,T}, ::Val...) (src/gemm.jl:244-246),
+    fastmul!(::MMatrix...) (src/kernels.jl:202-208), kernel!/initkernel!(pD, pA,
+    pX, ::Kernel{Mk,Pk,stride_AD,stride_X,N}[, pf]) (src/kernels.jl:239-241,
+    273-275) with Kernel(Mk, N, Pk, stride_AD, stride_X) (src/kernel_structure.jl:
+    8-9), methods of jBLAS's own functions, and device-array (ROCMatrix) methods."""
+    src = open(os.path.join(JL, "src", "JBlasAMD.jl")).read()
+    ext = {f: open(os.path.join(JL, "ext", f)).read() for f in os.listdir(os.path.join(JL, "ext"))}
+    sa, jb_, amd = ext["JBlasAMDStaticArraysExt.jl"], ext["JBlasAMDjBLASExt.jl"], ext["JBlasAMDAMDGPUExt.jl"]
+    mm = r"\(D::MMatrix\{M,P,T\}, A::MMatrix\{M,N,T\}, X::MMatrix\{N,P,T\}"
+    assert re.search(r"function JBlasAMD\.jmul!" + mm + r",\s*::Val = Val\(7\), ::Val = Val\(7\), ::Val = Val\(3\)", sa)
+    assert re.search(r"function JBlasAMD\.fastmul!" + mm, sa)
+    assert re.search(r"function jBLAS\.jmul!" + mm, jb_) and re.search(r"function jBLAS\.fastmul!" + mm, jb_)
+    for fn in ("kernel!", "initkernel!"):
+        assert re.search(r"function " + re.escape(fn) + r"\(pD::Ptr\{Float64\}, pA::Ptr\{Float64\}, pX::Ptr\{Float64\}, "
+                         r"::Kernel\{M\u2096,P\u2096,stride_AD,stride_X,N\}", src)
+        assert re.search(r"jBLAS\." + re.escape(fn) + r"\(pD::Ptr\{Float64\}.*\n\s*::jBLAS\.Kernel\{", jb_)
+        assert re.search(r"function JBlasAMD\." + re.escape(fn) + r"\(D::ROCMatrix\{Float64\}", amd)
+    assert "struct Kernel{M\u2096,P\u2096,stride_AD,stride_X,N} end" in src
+    assert "Kernel(M\u2096, N, P\u2096, stride_AD, stride_X) = Kernel{M\u2096,P\u2096,stride_AD,stride_X,N}()" in src
+    assert re.search(r"function JBlasAMD\.jmul!\(D::ROCMatrix\{Float64\}", amd)
+    assert re.search(r"function JBlasAMD\.gemm!\(.*A::ROCMatrix\{Float64\}", amd)
+    proj = open(os.path.join(JL, "Project.toml")).read()
+    for name in ("JBlasAMDStaticArraysExt", "JBlasAMDjBLASExt", "JBlasAMDAMDGPUExt", "JBlasAMDChainRulesCoreExt"):
+        assert name in proj and name + ".jl" in ext
 
 
 def test_version_and_error_string():
