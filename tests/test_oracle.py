@@ -5,6 +5,7 @@ index, a transposed operand, fma-vs-plain first step) fails it; none of them
 retypes the oracle's own formula.
 """
 import math
+import os
 from fractions import Fraction
 
 import numpy as np
@@ -12,6 +13,8 @@ import pytest
 
 from oracle import oracle as O
 from jblas_amd import generators as G
+
+ROOT_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def bf16_round(x):
@@ -506,3 +509,18 @@ def test_blocking_structure_table():
     assert O.blocking_structure(256, 256, 256, 8, cs, 64, 32) == (((40, 132, 5), (256, 256, 256), (256, 256, 256)), 1)
     assert O.blocking_structure(1024, 1024, 1024, 8, cs, 64, 32) == (((40, 132, 5), (400, 132, 395), (1024, 1024, 1024)), 2)
     assert O.blocking_structure(8, 8, 8, 8, cs, 64, 32) == (((8, 8, 5), (8, 8, 8), (8, 8, 8)), 0)
+
+
+def test_oracle_under_address_and_ub_sanitizers():
+    """Every entry point of the checker runs clean under AddressSanitizer and
+    UndefinedBehaviorSanitizer (oracle/asan_check.c: exact-size buffers, fringe
+    shapes, both transposes, strided leading dimensions, all slice counts)."""
+    import subprocess
+
+    here = os.path.join(ROOT_DIR, "oracle")
+    subprocess.run(["make", "-s", "-C", here, "asan"], check=True)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
+    r = subprocess.run([os.path.join(here, "_build", "oracle_asan")], capture_output=True, text=True, env=env,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ran clean" in r.stdout
