@@ -139,7 +139,8 @@ def native_and_boundary(torch, jb):
     * jmul! at 256^3 (the reference's CPU-runnable case): device-resident
       call and the host-pointer call (operands staged over PCIe);
     * the host-pointer DGEMM at 8192^3 (PCIe-inclusive: A and B copied in,
-      C copied out), native and split-BF16 s = 6."""
+      C copied out), native and split-BF16 s = 6;
+    * torch's BF16 and FP64 GEMM at 8192^3 (vendor yardsticks)."""
     import numpy as np
 
     out = {"c2_native_tflops": {}}
@@ -193,6 +194,18 @@ def native_and_boundary(torch, jb):
         ms = (time.perf_counter() - t0) / 2 * 1e3
         host[meth] = {"ms": round(ms, 2), "tflops": round(2.0 * n ** 3 / ms / 1e9, 2)}
     out["host_pointer_8192_pcie_inclusive"] = host
+    # vendor yardsticks on the same device: torch (hipBLASLt / rocBLAS) BF16 and
+    # FP64 GEMM at 8192^3, for the BF16 rate the slice-pair kernel reaches and
+    # for the native FP64 kernel
+    vendor = {}
+    for name, dt in (("bf16", torch.bfloat16), ("fp64", torch.float64)):
+        X = torch.randn(n, n, device="cuda").to(dt)
+        Y = torch.randn(n, n, device="cuda").to(dt)
+        Z = torch.matmul(X, Y)
+        ms = timed(torch, lambda: torch.matmul(X, Y, out=Z), 5)
+        vendor[f"torch_{name}_8192_tflops"] = round(2.0 * n ** 3 / ms / 1e9, 1)
+        del X, Y, Z
+    out["vendor_gemm_8192"] = vendor
     return out
 
 
