@@ -38,8 +38,8 @@ def parse():
     ap.add_argument("--nslices-int8", type=int, default=7, help="INT8 slices (8 bits each)")
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline and the oracle check")
     ap.add_argument("--skip-table", action="store_true", help="omit the per-slice-count table")
-    ap.add_argument("--only", choices=["ozaki", "native", "ozaki_int8"], default=None,
-                    help="run only one method (for profiler runs)")
+    ap.add_argument("--only", choices=["ozaki", "native", "ozaki_int8", "grouped"], default=None,
+                    help="run only one method, or only the batched and rrule workloads (for profiler runs)")
     ap.add_argument("--skip-grouped", action="store_true",
                     help="omit the batched (1024 uneven problems) and rrule (4096x2048x4096) measurements")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -104,6 +104,14 @@ def grouped_and_rrule(torch, jb):
             plan.destroy()
             res[f"{phase}_ms"] = round(ms, 4)
             res[f"{phase}_tflops"] = round(flops / ms / 1e9, 3)
+        # the grouped kernels against the matrix-core peak of the arithmetic they
+        # run: FP64 MFMA (78.6 TFLOP/s), or BF16 MFMA for 3xBF16 (2500 TFLOP/s;
+        # six BF16 products per FP32 product)
+        best = max(res["static_tflops"], res["tuned_tflops"])
+        ach, peak = (best, 78.6) if method == "f64" else (6 * best, 2500.0)
+        res["roofline"] = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+                           "frac": round(ach / peak, 4),
+                           "dtype": "fp64" if method == "f64" else "bf16 (6 products per fp32 product)"}
         torch.cuda.synchronize()
         # host copies of a few problems (operands, result) for the per-problem error check
         samples[method] = [(keep[3 * i].cpu().numpy().T, keep[3 * i + 1].cpu().numpy().T,
@@ -302,6 +310,11 @@ def main():
     pairs, pairs8 = s * (s + 1) // 2, s8 * (s8 + 1) // 2
     log(f"device {info['arch']} CUs={info['compute_units']} clock={info['clock_khz'] / 1e6:.2f}GHz "
         f"peak fp64={info['peak_fp64_tflops']:.1f} bf16={info['peak_bf16_tflops']:.1f} TF")
+
+    if args.only == "grouped":
+        extra, _ = grouped_and_rrule(torch, jb)
+        print(json.dumps({"grouped_and_rrule": extra}), flush=True)
+        return
 
     if args.copy_calibration > 0:
         src = torch.ones(args.copy_calibration * 2 ** 20 // 8, dtype=torch.float64, device="cuda")

@@ -305,12 +305,24 @@ int jb_autotune_grouped(int method, int64_t count, const jb_gemm_desc* descs, in
 /* Persist / restore the table as JSON (keyed by device, arch, method, bucket). */
 int jb_autotune_save(const char* path);
 int jb_autotune_load(const char* path);
-/* Query the choice for one problem shape: tile id (0 = 64x64 MFMA tile, 1 = 32x32
- * MFMA tile, 2 = the whole problem in one work item on the vector ALUs for
- * m, n, k <= 64 -- fastmul!'s form, src/kernels.jl:202-208) and split-K. */
+/* Query the choice for one problem shape on its own: tile id (0 = 64x64 MFMA
+ * tile, 1 = 32x32 MFMA tile, 2 = the whole problem in one work item on the
+ * vector ALUs for m, n, k <= 64 -- fastmul!'s form, src/kernels.jl:202-208 --,
+ * 3 = 128x128 MFMA tile, FP64 only, run by a second persistent kernel) and
+ * split-K, from the table or the static rule.  A plan may raise the split of
+ * statically chosen problems while the group has fewer work items than the
+ * device holds resident workgroups (jb_grouped_choices reports the result). */
 int jb_autotune_lookup(int method, int64_t m, int64_t n, int64_t k, int* tile, int* splitk);
 /* Clear the table (static defaults apply again). */
 int jb_autotune_reset(void);
+/* The choice a plan of this group makes now, per problem (arrays of `count`):
+ * tile id (as above; -1 for a problem with m or n == 0), split count, and k per
+ * split (a multiple of 32: split s sums k-values [s*kchunk, min(k, (s+1)*kchunk)),
+ * and the splits' partial results are added to C in split order).  No launch.
+ * No counterpart in the reference (its blocking is static); for tests and
+ * callers that need to reproduce the grouped launch's summation order. */
+int jb_grouped_choices(int method, int64_t count, const jb_gemm_desc* descs, int* tile, int* splitk,
+                       int64_t* kchunk);
 
 /* ---- differentiation (ChainRulesCore.rrule of gemm!) ---------------------------- */
 /* For C = alpha*op(A)*op(B) + beta*C_in and upstream gradient dC (m x n):

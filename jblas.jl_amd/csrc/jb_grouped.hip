@@ -38,11 +38,22 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
 constexpr int GNT = 256;  // 4 waves
+
+// Operand and result pointers of a problem come from its descriptor in memory,
+// so the compiler cannot infer their address space: without this cast every
+// access is a flat access, which also counts against the LDS wait counter
+// (an LDS wait then waits for the operand prefetch too).  Global-space
+// pointers give global loads and stores (and scalar-base addressing).
+template <typename T> using gptr = __attribute__((address_space(1))) T*;
+template <typename T> __device__ __forceinline__ gptr<T> as_global(T* p) { return (gptr<T>)p; }
 // Work-item configurations: 0 = 64x64 MFMA tile, 1 = 32x32 MFMA tile, 2 = the
-// whole problem on the vector ALUs (m, n, k <= SMALL_MAX, one split).
-constexpr int NCFG = 3;
-constexpr int CFG_BM[NCFG] = {64, 32, 64};
+// whole problem on the vector ALUs (m, n, k <= SMALL_MAX, one split), 3 =
+// 128x128 MFMA tile (FP64; run by a second persistent kernel whose register
+// budget is the large tile's, so the others keep four workgroups per CU).
+constexpr int NCFG = 4;
+constexpr int CFG_BM[NCFG] = {64, 32, 64, 128};
 constexpr int CFG_SMALL = 2;
+constexpr int CFG_LARGE = 3;
 constexpr int SMALL_MAX = 64;
 
 // Device descriptor of one problem.
@@ -87,7 +98,7 @@ struct GLoader {
   __device__ explicit GLoader(int t) : tid(t) {}
   static __device__ __forceinline__ int row_of(int e) { return RC ? (e % R) : (e / BK); }
   static __device__ __forceinline__ int k_of(int e) { return RC ? (e / R) : (e % BK); }
-  __device__ __forceinline__ void load(const T* __restrict__ P, int64_t ld, int64_t r0, int64_t k0, int64_t Rtot,
+  __device__ __forceinline__ void load(gptr<const T> P, int64_t ld, int64_t r0, int64_t k0, int64_t Rtot,
                                        int64_t Kend) {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
@@ -95,6 +106,17 @@ struct GLoader {
       const int64_t gr = r0 + row_of(e), gk = k0 + k_of(e);
       v[i] = (gr < Rtot && gk < Kend) ? (RC ? P[gr + gk * ld] : P[gk + gr * ld]) : (NEGZ ? T(-0.0) : T(0));
     }
+  }
+  // the same with the vector-loader interface (element loads in every mode)
+  static __device__ __forceinline__ int mode(bool, bool, bool) { return 0; }
+  __device__ __forceinline__ void load(gptr<const T> P, int64_t ld, int64_t r0, int64_t k0, int64_t Rtot,
+                                       int64_t Kend, int) {
+    load(P, ld, r0, k0, Rtot, Kend);
+  }
+  template <int STRIDE, bool SWZ>
+  __device__ __forceinline__ void store(T* S) const {
+    static_assert(!SWZ, "element staging is not swizzled");
+    store_kmajor<STRIDE>(S);
   }
   // k-major [kk][r] (MFMA fp64/fp32 operands)
   template <int STRIDE>
@@ -105,6 +127,104 @@ struct GLoader {
       S[k_of(e) * STRIDE + row_of(e)] = v[i];
     }
   }
+};
+
+// Staging of one R x 16 FP64 operand tile in 16-byte vectors (two doubles
+// along the contiguous index), k range [k0, Kend):
+//   RC (rows contiguous): vector i of thread t holds rows 2 (t % (R/2)) + {0,1}
+//                         of k-row t / (R/2) + (2 GNT / R) i
+//   otherwise:            vector i holds k 2 (t % 8) + {0,1} of row t / 8 + 32 i
+// Load modes (uniform per work item and stage):
+//   LD_FULL  the tile lies inside the operand, 16-byte loads are aligned;
+//   LD_VPRED aligned, and the operand's extent along the vector is even, so a
+//            vector (even start) is wholly inside or wholly outside: one
+//            predicate per vector;
+//   LD_ELEM  each element loaded on its own.
+// Elements outside the operand read as PADV.
+enum { LD_FULL = 0, LD_VPRED = 1, LD_ELEM = 2 };
+template <int R, bool RC, bool NEGZ>
+struct GVecLoader {
+  static constexpr int BK = 16;
+  static constexpr int PER = R * BK / (2 * GNT);
+  static_assert(PER >= 1 && PER * 2 * GNT == R * BK, "tile must split into whole vectors per thread");
+  static constexpr double PADV = NEGZ ? -0.0 : 0.0;
+  typedef double V __attribute__((ext_vector_type(2)));
+  V v[PER];
+  int tid;
+  __device__ explicit GVecLoader(int t) : tid(t) {}
+  __device__ __forceinline__ int r_of(int i) const { return RC ? 2 * (tid % (R / 2)) : tid / 8 + (2 * GNT / BK) * i; }
+  __device__ __forceinline__ int k_of(int i) const { return RC ? tid / (R / 2) + (2 * GNT / R) * i : 2 * (tid % 8); }
+  // the mode of a stage: vec = aligned vectors, even = even extent along the vector
+  static __device__ __forceinline__ int mode(bool vec, bool even, bool full) {
+    return vec && full ? LD_FULL : (vec && even ? LD_VPRED : LD_ELEM);
+  }
+  // FULLPATH = false: LD_FULL is served by the predicated path (fewer live
+  // registers in a kernel at its register limit)
+  template <bool FULLPATH = true>
+  __device__ __forceinline__ void load(gptr<const double> P, int64_t ld, int64_t r0, int64_t k0, int64_t Rtot,
+                                       int64_t Kend, int md) {
+    if (FULLPATH && md == LD_FULL) {
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int64_t gr = r0 + r_of(i), gk = k0 + k_of(i);
+        v[i] = *(gptr<const V>)(RC ? P + gr + gk * ld : P + gk + gr * ld);
+      }
+    } else if (md != LD_ELEM) {
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int64_t gr = r0 + r_of(i), gk = k0 + k_of(i);
+        v[i] = (gr < Rtot && gk < Kend) ? *(gptr<const V>)(RC ? P + gr + gk * ld : P + gk + gr * ld)
+                                        : V{PADV, PADV};
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int64_t gr = r0 + r_of(i), gk = k0 + k_of(i);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int64_t r = RC ? gr + e : gr, k = RC ? gk : gk + e;
+          v[i][e] = (r < Rtot && k < Kend) ? (RC ? P[r + k * ld] : P[k + r * ld]) : PADV;
+        }
+      }
+    }
+  }
+  // k-major [k][row] at row stride STRIDE; SWZ (k-contiguous operands): row r
+  // of k-row k sits at column r ^ lswz(k), so the two k-rows of one store
+  // instruction's lanes fall in distinct banks.
+  template <int STRIDE, bool SWZ>
+  __device__ __forceinline__ void store(double* S) const {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int r = r_of(i), k = k_of(i);
+      if (RC) {
+        *reinterpret_cast<V*>(S + k * STRIDE + r) = v[i];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 2; ++e) S[(k + e) * STRIDE + (SWZ ? (r ^ lswz(k + e)) : r)] = v[i][e];
+      }
+    }
+  }
+  static __device__ __forceinline__ int lswz(int k) { return 4 * ((k >> 1) & 7); }
+};
+
+// The vector mode conditions of one operand: `vec` (16-byte aligned base, even
+// leading dimension) and `even` (even extent along the vector: rows for RC,
+// the k range end otherwise).
+template <bool RC>
+__device__ __forceinline__ void vec_conditions(const void* P, int64_t ld, int64_t rows, int64_t kend, bool& vec,
+                                               bool& even) {
+  vec = ((reinterpret_cast<uintptr_t>(P) & 15) == 0) && (ld % 2 == 0);
+  even = RC ? rows % 2 == 0 : kend % 2 == 0;
+}
+
+// Operand staging of the MFMA tiles: 16-byte vectors for FP64, elements for FP32.
+template <typename T, int R, bool RC, bool NEGZ>
+struct TileLoaderOf {
+  typedef GLoader<T, R, 16, RC, NEGZ> type;
+};
+template <int R, bool RC, bool NEGZ>
+struct TileLoaderOf<double, R, RC, NEGZ> {
+  typedef GVecLoader<R, RC, NEGZ> type;
 };
 
 typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
@@ -132,7 +252,7 @@ __device__ __forceinline__ void put_result(const GProb& P, int tm, int tn, int s
   if (P.splitk == 1) {
     const int64_t m = (int64_t)tm * BM + i, n = (int64_t)tn * BN + j;
     if (m < P.m && n < P.n) {
-      T* C = static_cast<T*>(P.C);
+      gptr<T> C = as_global(static_cast<T*>(P.C));
       T c;
       if (P.k == 0) {  // no products (k == 0 or alpha == 0): C = beta*C, 0 for beta == 0
         c = (T)P.beta != T(0) ? (T)P.beta * C[m + n * P.ldc] : T(0);
@@ -144,7 +264,7 @@ __device__ __forceinline__ void put_result(const GProb& P, int tm, int tn, int s
     }
   } else {
     const int tile = tm * P.tiles_n + tn;
-    T* slab = static_cast<T*>(P.part) + ((int64_t)tile * P.splitk + split) * (BM * BN);
+    gptr<T> slab = as_global(static_cast<T*>(P.part)) + ((int64_t)tile * P.splitk + split) * (BM * BN);
     slab[i + j * BM] = v;
   }
 }
@@ -161,8 +281,8 @@ __device__ void splitk_combine(const GProb& P, int tm, int tn, int* counters, in
   __syncthreads();
   if (*sflag != P.splitk - 1) return;  // not the last split of this tile
   __threadfence();
-  const T* slab = static_cast<const T*>(P.part) + ((int64_t)tile * P.splitk) * (BM * BN);
-  T* C = static_cast<T*>(P.C);
+  gptr<const T> slab = as_global(static_cast<const T*>(P.part)) + ((int64_t)tile * P.splitk) * (BM * BN);
+  gptr<T> C = as_global(static_cast<T*>(P.C));
   const T alpha = (T)P.alpha, beta = (T)P.beta;
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
   for (int e = threadIdx.x; e < BM * BN; e += GNT) {
@@ -194,10 +314,21 @@ __device__ void native_tile(const GProb& P, int tm, int tn, int split, char* sme
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
   const int64_t kb = (int64_t)split * P.kchunk;
   const int64_t ke = min(P.k, kb + P.kchunk);
-  const T* A = static_cast<const T*>(P.A);
-  const T* B = static_cast<const T*>(P.B);
-  GLoader<T, BM, BK, !TA, sizeof(T) == 8> la(tid);  // op(A) rows contiguous when A is not transposed
-  GLoader<T, BN, BK, TB> lb(tid);   // op(B) columns contiguous when B is transposed
+  gptr<const T> A = as_global(static_cast<const T*>(P.A));
+  gptr<const T> B = as_global(static_cast<const T*>(P.B));
+  // op(A) rows contiguous when A is not transposed, op(B) columns when B is;
+  // FP64 operands travel in 16-byte vectors
+  typename TileLoaderOf<T, BM, !TA, sizeof(T) == 8>::type la(tid);
+  typename TileLoaderOf<T, BN, TB, false>::type lb(tid);
+  bool va, ea, vb, eb;
+  vec_conditions<!TA>(P.A, P.lda, P.m, ke, va, ea);
+  vec_conditions<TB>(P.B, P.ldb, P.n, ke, vb, eb);
+  const bool fa = m0 + BM <= P.m, fb = n0 + BN <= P.n;
+  auto load_stage = [&](int64_t k0) {
+    const bool fk = k0 + BK <= ke;
+    la.load(A, P.lda, m0, k0, P.m, ke, la.mode(va, ea, fa && fk));
+    lb.load(B, P.ldb, n0, k0, P.n, ke, lb.mode(vb, eb, fb && fk));
+  };
   // FP64 chains start from -0.0 and A's k-padding is -0.0: the first step is
   // exactly the plain product and padding products leave the chain unchanged,
   // so each split is the reference recurrence bit for bit, zeros included.
@@ -209,18 +340,14 @@ __device__ void native_tile(const GProb& P, int tm, int tn, int split, char* sme
     for (int j = 0; j < JN; ++j) acc[i][j] = acc_t{acc0, acc0, acc0, acc0};
   const int nkt = (int)((ke - kb + BK - 1) / BK);
   __syncthreads();  // LDS may still be read by the previous work item
-  la.load(A, P.lda, m0, kb, P.m, ke);
-  lb.load(B, P.ldb, n0, kb, P.n, ke);
-  la.template store_kmajor<SA>(As);
-  lb.template store_kmajor<SB>(Bs);
+  load_stage(kb);
+  la.template store<SA, false>(As);
+  lb.template store<SB, false>(Bs);
   // Stage kt+1 travels in registers during stage kt: loaded right after stage
   // kt went to LDS, stored to the other buffer before stage kt's last MFMA
   // group, so a load has a whole stage to arrive and the LDS writes finish
   // under MFMAs.
-  if (nkt > 1) {
-    la.load(A, P.lda, m0, kb + BK, P.m, ke);
-    lb.load(B, P.ldb, n0, kb + BK, P.n, ke);
-  }
+  if (nkt > 1) load_stage(kb + BK);
   __syncthreads();
   for (int kt = 0; kt < nkt; ++kt) {
     const int cur = kt & 1;
@@ -238,12 +365,9 @@ __device__ void native_tile(const GProb& P, int tm, int tn, int split, char* sme
 #pragma unroll
         for (int j = 0; j < JN; ++j) acc[i][j] = GMfma<T>::mma(b[j], a[i], acc[i][j]);
       if (t == BK / 4 - 2 && kt + 1 < nkt) {
-        la.template store_kmajor<SA>(As + (cur ^ 1) * BK * SA);
-        lb.template store_kmajor<SB>(Bs + (cur ^ 1) * BK * SB);
-        if (kt + 2 < nkt) {
-          la.load(A, P.lda, m0, kb + (int64_t)(kt + 2) * BK, P.m, ke);
-          lb.load(B, P.ldb, n0, kb + (int64_t)(kt + 2) * BK, P.n, ke);
-        }
+        la.template store<SA, false>(As + (cur ^ 1) * BK * SA);
+        lb.template store<SB, false>(Bs + (cur ^ 1) * BK * SB);
+        if (kt + 2 < nkt) load_stage(kb + (int64_t)(kt + 2) * BK);
       }
     }
     __syncthreads();
@@ -256,6 +380,128 @@ __device__ void native_tile(const GProb& P, int tm, int tn, int split, char* sme
       for (int r = 0; r < 4; ++r)
         put_result<T, BM, BN>(P, tm, tn, split, wm0 + 16 * i + l16, wn0 + 16 * j + GMfma<T>::prow(r, g), acc[i][j][r]);
   splitk_combine<T, BM, BN>(P, tm, tn, counters, sflag);
+}
+
+// 128x128 FP64 tile: four waves of 64x64 (4x4 blocks of v_mfma_f64_16x16x4f64),
+// 16 k per LDS stage, two LDS buffers, the next stage's loads one stage ahead
+// in registers.  Same chain semantics as native_tile: chains start from -0.0,
+// A's padding is -0.0, each k-split is one fma chain in increasing k.
+template <bool TA, bool TB>
+__device__ void large_tile_f64(const GProb& P, int tm, int tn, int split, char* smem, int* counters, int* sflag) {
+  constexpr int BM = 128, BK = 16, ST = BM + 16;  // row stride 144: lane groups g, g+1 start 32 banks apart
+  constexpr int IM = 4, JN = 4;
+  constexpr bool SWA = TA, SWB = !TB;  // k-contiguous operands are stored swizzled
+  double* const As = reinterpret_cast<double*>(smem);
+  double* const Bs = As + 2 * BK * ST;
+  const int tid = work_item_tid();
+  const int lane = tid % 64, wave = tid / 64;
+  const int wm0 = (wave % 2) * 64, wn0 = (wave / 2) * 64;
+  const int g = lane / 16, l16 = lane % 16;
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BM;
+  const int64_t kb = (int64_t)split * P.kchunk;
+  const int64_t ke = min(P.k, kb + P.kchunk);
+  gptr<const double> A = as_global(static_cast<const double*>(P.A));
+  gptr<const double> B = as_global(static_cast<const double*>(P.B));
+  GVecLoader<BM, !TA, true> la(tid);
+  GVecLoader<BM, TB, false> lb(tid);
+  // 16-byte loads: aligned bases and even leading dimensions (k0 and the
+  // vectors' start rows are even)
+  bool va, ea, vb, eb;
+  vec_conditions<!TA>(P.A, P.lda, P.m, ke, va, ea);
+  vec_conditions<TB>(P.B, P.ldb, P.n, ke, vb, eb);
+  const bool fa = m0 + BM <= P.m, fb = n0 + BM <= P.n;
+  d4 acc[IM][JN];
+#pragma unroll
+  for (int i = 0; i < IM; ++i)
+#pragma unroll
+    for (int j = 0; j < JN; ++j) acc[i][j] = d4{-0.0, -0.0, -0.0, -0.0};
+  const int nkt = (int)((ke - kb + BK - 1) / BK);
+  auto load_stage = [&](int kt) {
+    const int64_t k0 = kb + (int64_t)kt * BK;
+    const bool fk = k0 + BK <= ke;
+    if (fa && fb && fk && va && vb) {
+      la.template load<true>(A, P.lda, m0, k0, P.m, ke, LD_FULL);
+      lb.template load<true>(B, P.ldb, n0, k0, P.n, ke, LD_FULL);
+    } else {
+      la.template load<false>(A, P.lda, m0, k0, P.m, ke, la.mode(va, ea, false));
+      lb.template load<false>(B, P.ldb, n0, k0, P.n, ke, lb.mode(vb, eb, false));
+    }
+  };
+  __syncthreads();  // LDS may still be read by the previous work item
+  load_stage(0);
+  la.template store<ST, SWA>(As);
+  lb.template store<ST, SWB>(Bs);
+  if (nkt > 1) load_stage(1);
+  __syncthreads();
+  // Swizzled column of (w0 + 16 i + l16) in k-row 4t + g: 16 ((w0/16 + i) ^ (t >> 1))
+  // + (l16 ^ (8 (t & 1) + 4 (g >> 1))) for t < 4 (w0 a multiple of 16).
+  const int sw0 = l16 ^ (4 * (g >> 1)), sw1 = l16 ^ (8 + 4 * (g >> 1));
+  const int bm0 = wm0 / 16, bn0 = wn0 / 16;
+  auto frags = [&](const double* as, const double* bs, int t, double (&a)[IM], double (&b)[JN]) {
+    const int swc = (t & 1) ? sw1 : sw0;
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+      a[i] = as[(4 * t + g) * ST + (SWA ? 16 * ((bm0 + i) ^ (t >> 1)) + swc : wm0 + 16 * i + l16)];
+#pragma unroll
+    for (int j = 0; j < JN; ++j)
+      b[j] = bs[(4 * t + g) * ST + (SWB ? 16 * ((bn0 + j) ^ (t >> 1)) + swc : wn0 + 16 * j + l16)];
+  };
+  for (int kt = 0; kt < nkt; ++kt) {
+    const int cur = kt & 1;
+    const double* as = As + cur * BK * ST;
+    const double* bs = Bs + cur * BK * ST;
+    double a[2][IM], b[2][JN];
+    frags(as, bs, 0, a[0], b[0]);
+#pragma unroll
+    for (int t = 0; t < BK / 4; ++t) {
+      if (t + 1 < BK / 4) frags(as, bs, t + 1, a[(t + 1) & 1], b[(t + 1) & 1]);
+#pragma unroll
+      for (int i = 0; i < IM; ++i)
+#pragma unroll
+        for (int j = 0; j < JN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[t & 1][j], a[t & 1][i], acc[i][j], 0, 0, 0);
+      if (t == BK / 4 - 2 && kt + 1 < nkt) {
+        la.template store<ST, SWA>(As + (cur ^ 1) * BK * ST);
+        lb.template store<ST, SWB>(Bs + (cur ^ 1) * BK * ST);
+        if (kt + 2 < nkt) load_stage(kt + 2);
+      }
+    }
+    __syncthreads();
+  }
+  // results: a 16-lane group holds 16 consecutive rows of one column
+  if (P.splitk == 1) {
+    gptr<double> C = as_global(static_cast<double*>(P.C));
+    const double alpha = P.alpha, beta = P.beta;
+#pragma unroll
+    for (int i = 0; i < IM; ++i) {
+      const int64_t m = m0 + wm0 + 16 * i + l16;
+      if (m >= P.m) continue;
+#pragma unroll
+      for (int j = 0; j < JN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t n = n0 + wn0 + 16 * j + GMfma<double>::prow(r, g);
+          if (n < P.n) {  // as put_result: k == 0 (or alpha == 0) gives beta*C, +0 for beta == 0
+            gptr<double> c = C + m + n * P.ldc;
+            double v;
+            if (P.k == 0) v = beta != 0.0 ? beta * *c : 0.0;
+            else v = beta != 0.0 ? fma(beta, *c, alpha * acc[i][j][r]) : alpha * acc[i][j][r];
+            *c = v;
+          }
+        }
+    }
+  } else {
+    gptr<double> slab =
+        as_global(static_cast<double*>(P.part)) + ((int64_t)(tm * P.tiles_n + tn) * P.splitk + split) * (BM * BM);
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+#pragma unroll
+      for (int j = 0; j < JN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          slab[(wm0 + 16 * i + l16) + (wn0 + 16 * j + GMfma<double>::prow(r, g)) * BM] = acc[i][j][r];
+    splitk_combine<double, BM, BM>(P, tm, tn, counters, sflag);
+  }
 }
 
 // LDS rows of the 3xBF16 tile hold BK = 32 BF16 (64 bytes, four 16-byte chunks,
@@ -280,7 +526,7 @@ struct GRun4 {
   __device__ explicit GRun4(int t) : tid(t) {}
   __device__ __forceinline__ int row(int q) const { return RC ? tid % R : tid / (BK / 4) + (4 * GNT / BK) * q; }
   __device__ __forceinline__ int krun(int q) const { return RC ? tid / R + (GNT / R) * q : tid % (BK / 4); }
-  __device__ __forceinline__ void load(const float* __restrict__ P, int64_t ld, int64_t r0, int64_t k0, int64_t Rtot,
+  __device__ __forceinline__ void load(gptr<const float> P, int64_t ld, int64_t r0, int64_t k0, int64_t Rtot,
                                        int64_t Kend) {
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
@@ -330,8 +576,8 @@ __device__ void bf16x3_tile(const GProb& P, int tm, int tn, int split, char* sme
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
   const int64_t kb = (int64_t)split * P.kchunk;
   const int64_t ke = min(P.k, kb + P.kchunk);
-  const float* A = static_cast<const float*>(P.A);
-  const float* B = static_cast<const float*>(P.B);
+  gptr<const float> A = as_global(static_cast<const float*>(P.A));
+  gptr<const float> B = as_global(static_cast<const float*>(P.B));
   GRun4<BM, BK, !TA> la(tid);
   GRun4<BN, BK, TB> lb(tid);
   f4 acc[2][IM][JN];
@@ -426,8 +672,8 @@ __device__ __forceinline__ void small_tile(const GProb& P, char* smem) {
   const int tid = work_item_tid();
   const int lane = tid % 64, wave = tid / 64;
   const int64_t m = P.m, n = P.n, k = P.k;
-  const T* __restrict__ A = static_cast<const T*>(P.A);
-  const T* __restrict__ B = static_cast<const T*>(P.B);
+  gptr<const T> A = as_global(static_cast<const T*>(P.A));
+  gptr<const T> B = as_global(static_cast<const T*>(P.B));
   // stage op(B): element e = tid + 256 q is (p, j) with the contiguous index fastest
   constexpr int NQ = 64 * 64 / GNT;  // 16 elements per thread, staged 8 at a time
 #pragma unroll
@@ -447,7 +693,7 @@ __device__ __forceinline__ void small_tile(const GProb& P, char* smem) {
     }
   }
   const int64_t i = lane < m ? lane : m - 1;
-  const T* arow = TA ? A + i * P.lda : A + i;  // A(i, p) = arow[p * astep]
+  gptr<const T> arow = TA ? A + i * P.lda : A + i;  // A(i, p) = arow[p * astep]
   const int64_t astep = TA ? 1 : P.lda;
   T a[2][AC];
 #pragma unroll
@@ -456,11 +702,11 @@ __device__ __forceinline__ void small_tile(const GProb& P, char* smem) {
   const int j0 = SC * wave;
   if (j0 >= n || k == 0) {
     if (k == 0 && j0 < n && lane < m) {  // alpha == 0 or k == 0: C = beta*C, 0 for beta == 0
-      T* C = static_cast<T*>(P.C);
+      gptr<T> C = as_global(static_cast<T*>(P.C));
 #pragma unroll
       for (int c = 0; c < SC; ++c)
         if (j0 + c < n) {
-          T* cp = C + i + (j0 + c) * P.ldc;
+          gptr<T> cp = C + i + (j0 + c) * P.ldc;
           *cp = (T)P.beta != T(0) ? (T)P.beta * *cp : T(0);
         }
     }
@@ -510,7 +756,7 @@ __device__ __forceinline__ void small_tile(const GProb& P, char* smem) {
     }
   }
   if (lane >= m) return;
-  T* C = static_cast<T*>(P.C);
+  gptr<T> C = as_global(static_cast<T*>(P.C));
   const T alpha = (T)P.alpha, beta = (T)P.beta;
 #pragma unroll
   for (int c = 0; c < SC; ++c) {
@@ -561,26 +807,70 @@ __device__ __forceinline__ void dispatch_tile(const GProb& P, int combo, int tm,
   else run_small<KIND>(P, combo, smem);
 }
 
+// Work queues of one persistent launch, one per XCD: queue x holds items
+// [qdesc[x], qdesc[x] + qdesc[NXCD + x]) of the launch's item array and is
+// drained through counter qnext[x].  Workgroups are dispatched to the XCDs
+// round-robin, so workgroup b pops from queue b % NXCD first: the tiles that
+// share operand panels were put in one queue and meet in one XCD's L2.  Once
+// its queue is drained a workgroup takes items from the others in turn.
+// Returns the item index, or -1 after every queue was found drained (each
+// workgroup then leaves; an item is taken exactly once, by the atomic on its
+// queue's counter).
+constexpr int NXCD = 8;
+__device__ __forceinline__ int pop_item(const int* __restrict__ qdesc, int* __restrict__ qnext, int& q, int& tries,
+                                        int* s_item) {
+  for (;;) {
+    if (threadIdx.x == 0) *s_item = atomicAdd(qnext + q, 1);
+    __syncthreads();
+    // uniform across the wave: descriptor loads become scalar loads into SGPRs
+    const int it = __builtin_amdgcn_readfirstlane(*s_item);
+    __syncthreads();
+    if (it < qdesc[NXCD + q]) return qdesc[q] + it;
+    if (++tries == NXCD) return -1;
+    q = (q + 1) % NXCD;
+  }
+}
+
 // KIND 0: FP64 native, 1: FP32 native, 2: FP32 3xBF16.
 template <int KIND>
 __global__ __launch_bounds__(GNT, 2) void grouped_kernel(const GProb* __restrict__ probs, const GItem* __restrict__ items,
-                                                         int nitems, int* __restrict__ next, int* __restrict__ counters) {
+                                                         const int* __restrict__ qdesc, int* __restrict__ qnext,
+                                                         int* __restrict__ counters) {
   constexpr int SMEM = (KIND == 2) ? (2 * 3 * (64 + 64) * 32 * 2) : (2 * 16 * (64 + 8) * 2 * (KIND == 0 ? 8 : 4));
   __shared__ __attribute__((aligned(16))) char smem[SMEM];
   static_assert(SMEM >= 64 * (64 + 16 / (KIND == 0 ? 8 : 4)) * (KIND == 0 ? 8 : 4), "small path stages op(B) in LDS");
   __shared__ int s_item, s_flag;
+  int q = blockIdx.x % NXCD, tries = 0;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(next, 1);
-    __syncthreads();
-    // uniform across the wave: descriptor loads become scalar loads into SGPRs
-    const int it = __builtin_amdgcn_readfirstlane(s_item);
-    __syncthreads();
-    if (it >= nitems) break;  // every workgroup leaves once the queue is drained
+    const int it = pop_item(qdesc, qnext, q, tries, &s_item);
+    if (it < 0) break;
     const GItem w = items[it];
     const GProb P = probs[w.prob];
     const int tm = w.tile / P.tiles_n, tn = w.tile % P.tiles_n;
     const int combo = (P.ta ? 2 : 0) + (P.tb ? 1 : 0);
     dispatch_tile<KIND>(P, combo, tm, tn, w.split, smem, counters, &s_flag);
+  }
+}
+
+// The 128x128 tiles (FP64) of a group with one transpose combination, pulled
+// from their own queue (one kernel per combination: a single kernel holding all
+// four tile variants exceeds the register budget of two workgroups per CU).
+template <bool TA, bool TB>
+__global__ __launch_bounds__(GNT, 2) void grouped_large_kernel(const GProb* __restrict__ probs,
+                                                               const GItem* __restrict__ items,
+                                                               const int* __restrict__ qdesc, int* __restrict__ qnext,
+                                                               int* __restrict__ counters) {
+  constexpr int SMEM = 2 * 2 * 16 * (128 + 16) * 8;  // two buffers of A and B, 16 k-rows of stride 144
+  __shared__ __attribute__((aligned(16))) char smem[SMEM];
+  __shared__ int s_item, s_flag;
+  int q = blockIdx.x % NXCD, tries = 0;
+  for (;;) {
+    const int it = pop_item(qdesc, qnext, q, tries, &s_item);
+    if (it < 0) break;
+    const GItem w = items[it];
+    const GProb P = probs[w.prob];
+    const int tm = w.tile / P.tiles_n, tn = w.tile % P.tiles_n;
+    large_tile_f64<TA, TB>(P, tm, tn, w.split, smem, counters, &s_flag);
   }
 }
 
@@ -620,48 +910,71 @@ std::string bucket_key(int method, int64_t m, int64_t n, int64_t k) {
   return buf;
 }
 
-// Static defaults (no table entry): the largest tile that still yields a few
-// tiles, and split-K when the output is small but k is long.
-// Problems with m, n, k <= 64 take the small path (one work item, B staged in
-// LDS): on 1024 such problems it measured 43.0 us against 44.4 us for the
-// tiles in FP64 and 34.7 against 37.6 us in 3xBF16.
-void default_choice(int64_t m, int64_t n, int64_t k, int& cfg, int& splitk) {
+// Static defaults (no table entry).  Tile: the small path for m, n, k <= 64
+// (one work item, B staged in LDS: on 1024 such problems it measured 43.0 us
+// against 44.4 us for the tiles in FP64 and 34.7 against 37.6 us in 3xBF16);
+// otherwise the largest tile whose padded area is still mostly useful -- for
+// FP64 the 128x128 tile when both m and n reach 128 and at least 85 % of its
+// padded area is C, else 64x64 from max(m, n) >= 96, else 32x32.  Split-K is
+// not chosen per problem: plan_build splits the longest work items when the
+// whole group has fewer work items than the device holds workgroups (the
+// blocking decision that depends on the device, blocking_structure's role in
+// src/memory_management.jl:78-140).
+void default_choice(int kind, int64_t m, int64_t n, int64_t k, int& cfg, int& splitk) {
+  splitk = 1;
   if (m <= SMALL_MAX && n <= SMALL_MAX && k <= SMALL_MAX) {
     cfg = CFG_SMALL;
-    splitk = 1;
     return;
   }
-  const int64_t mn = std::max(m, n);
-  cfg = mn >= 96 ? 0 : 1;
-  const int bm = CFG_BM[cfg];
-  const int64_t tiles = ((m + bm - 1) / bm) * ((n + bm - 1) / bm);
-  splitk = 1;
-  while (splitk < 8 && tiles * splitk < 4 && k / (splitk * 2) >= 256) splitk *= 2;
+  if (kind == 0 && m >= 128 && n >= 128) {
+    const double padded = (double)((m + 127) / 128 * 128) * (double)((n + 127) / 128 * 128);
+    if ((double)m * (double)n >= 0.85 * padded) {
+      cfg = CFG_LARGE;
+      return;
+    }
+  }
+  cfg = std::max(m, n) >= 96 ? 0 : 1;
 }
 
 // A choice taken from a bucket (table, tuner trial, forced) may name the small
-// path for a problem that does not fit it: such a problem takes the 32x32 tile.
-void fit_choice(int64_t m, int64_t n, int64_t k, int& cfg, int& splitk) {
+// path for a problem that does not fit it: such a problem takes the 32x32 tile;
+// the 128x128 tile exists for FP64 only (others take 64x64).
+void fit_choice(int kind, int64_t m, int64_t n, int64_t k, int& cfg, int& splitk) {
   if (cfg == CFG_SMALL) {
     if (m <= SMALL_MAX && n <= SMALL_MAX && k <= SMALL_MAX) splitk = 1;
     else cfg = 1;
   }
+  if (cfg == CFG_LARGE && kind != 0) cfg = 0;
 }
 
-void choose(int method, int64_t m, int64_t n, int64_t k, int& cfg, int& splitk) {
+// Returns true when the choice is the static default (plan_build may then add split-K).
+bool choose(int method, int64_t m, int64_t n, int64_t k, int& cfg, int& splitk) {
+  const int kind = kind_of(method);
   auto it = g_table.find(bucket_key(method, m, n, k));
+  bool dflt = false;
   if (it != g_table.end()) {
     cfg = it->second.cfg;
     splitk = it->second.splitk;
   } else {
-    default_choice(m, n, k, cfg, splitk);
+    default_choice(kind, m, n, k, cfg, splitk);
+    dflt = true;
   }
-  fit_choice(m, n, k, cfg, splitk);
+  fit_choice(kind, m, n, k, cfg, splitk);
+  return dflt;
 }
 
 // ---------------------------------------------------------------------------
 // plans
 // ---------------------------------------------------------------------------
+constexpr int NLAUNCH = 5;  // persistent launches of a plan: the 128x128 kernel per transpose combination, the rest
+
+// Relative duration of one work item of problem g (MFMA work of one k-split
+// of one tile plus a fixed epilogue share).
+double item_cost(const GProb& g) {
+  const double bm = CFG_BM[g.cfg];
+  return bm * bm * (double)std::max<int64_t>(g.kchunk, 32) + 4096.0;
+}
+
 // forced choice for the autotuner's trial launches: cfg >= 0 forces one choice
 // on every problem; otherwise `over` (bucket -> choice) is consulted before the table
 struct Forced {
@@ -673,12 +986,15 @@ struct Forced {
 
 struct GroupedPlanImpl {
   int method = 0, kind = 0;
-  int nitems = 0, nprobs = 0, ncounters = 0, grid = 0;
-  void* dev = nullptr;  // probs | items | next+counters | partials
+  int nitems = 0, nprobs = 0, ncounters = 0;
+  // launches 0-3: the 128x128 kernel of transpose combination c; 4: the other items
+  int litems[NLAUNCH] = {0, 0, 0, 0, 0}, lgrid[NLAUNCH] = {0, 0, 0, 0, 0};
+  void* dev = nullptr;  // probs | items | queue descriptors | counters | partials
   size_t dev_bytes = 0;
   GProb* probs = nullptr;
   GItem* items = nullptr;
-  int* next = nullptr;  // next[0] = work counter, next[1..] = tile counters
+  int* qdesc = nullptr;  // per launch: NXCD queue offsets into items, then NXCD queue lengths
+  int* next = nullptr;   // per launch NXCD queue counters, then per-tile arrival counters (zeroed every launch)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t cap = nullptr;
@@ -719,18 +1035,20 @@ size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 // Resident workgroups per CU of the persistent kernel of `kind`, from the
 // device's register and LDS budget for that kernel (queried once per kind and
 // device, under the runtime lock; the persistent grid is this times the CU count).
+// kind 3: the 128x128 FP64 kernel.
 hipError_t grouped_occupancy(int kind, int* occ) {
-  static int cached[3] = {0, 0, 0};
+  static int cached[4] = {0, 0, 0, 0};
   static int cached_device = -1;
   if (cached_device != rt().device) {
-    cached[0] = cached[1] = cached[2] = 0;
+    cached[0] = cached[1] = cached[2] = cached[3] = 0;
     cached_device = rt().device;
   }
   if (cached[kind] == 0) {
     int n = 0;
     hipError_t e = kind == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<0>, GNT, 0)
                    : kind == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<1>, GNT, 0)
-                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<2>, GNT, 0);
+                   : kind == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<2>, GNT, 0)
+                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_large_kernel<true, true>, GNT, 0);
     if (e != hipSuccess) return e;
     cached[kind] = std::min(std::max(n, 1), 8);
   }
@@ -739,16 +1057,22 @@ hipError_t grouped_occupancy(int kind, int* occ) {
 }
 
 // Builds the device-side problem/work lists of a plan (no launch).
-int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& forced, GroupedPlanImpl& P) {
-  P.method = method;
-  P.kind = kind_of(method);
-  const size_t elt = P.kind == 0 ? 8 : 4;
-  std::vector<GProb> probs;
-  std::vector<std::pair<double, GItem>> work;
-  size_t part_elems = 0;
-  int counters = 0;
-  std::vector<size_t> part_off;
-  P.flops = 0;
+// Tile and split of every problem with m, n > 0 (in order), the plan's first
+// two passes: the table / tuner / static choice per problem, then the device
+// fill: while a persistent kernel (the 128x128 one, or the one for the other
+// tiles) has fewer work items than the device holds of its resident
+// workgroups (compute units x that kernel's occupancy), the split of its
+// statically chosen problem with the longest work items doubles, down to 256 k
+// per split.  Last, each split's k range is fixed.  flops: sum of 2 m n k.
+int plan_choices(int method, int64_t count, const jb_gemm_desc* d, const Forced& forced, std::vector<GProb>& probs,
+                 double* flops) {
+  const int kind = kind_of(method);
+  int occ = 0, occ_large = 1;
+  hipError_t e = grouped_occupancy(kind, &occ);
+  if (e == hipSuccess && kind == 0) e = grouped_occupancy(3, &occ_large);
+  if (e != hipSuccess) return hip_fail(e, "grouped plan: occupancy query");
+  std::vector<char> dflt;
+  *flops = 0;
   for (int64_t i = 0; i < count; ++i) {
     const jb_gemm_desc& x = d[i];
     if (x.m == 0 || x.n == 0) continue;
@@ -767,6 +1091,7 @@ int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& f
     g.beta = x.beta;
     g.ta = !(x.transA == 'N' || x.transA == 'n');
     g.tb = !(x.transB == 'N' || x.transB == 'n');
+    bool is_default = false;
     if (forced.cfg >= 0) {
       g.cfg = forced.cfg;
       g.splitk = forced.splitk;
@@ -780,50 +1105,137 @@ int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& f
         g.cfg = o->cfg;
         g.splitk = o->splitk;
       } else {
-        choose(method, g.m, g.n, g.k, g.cfg, g.splitk);
+        is_default = choose(method, g.m, g.n, g.k, g.cfg, g.splitk);
       }
     }
-    fit_choice(g.m, g.n, g.k, g.cfg, g.splitk);
+    fit_choice(kind, g.m, g.n, g.k, g.cfg, g.splitk);
     const int bm = CFG_BM[g.cfg];
     g.tiles_m = (int)((g.m + bm - 1) / bm);
     g.tiles_n = (int)((g.n + bm - 1) / bm);
-    // each split covers a multiple of 32 k-values; splits never come out empty
+    *flops += 2.0 * (double)g.m * (double)g.n * (double)x.k;
+    probs.push_back(g);
+    dflt.push_back(is_default && g.cfg != CFG_SMALL);
+  }
+  const int64_t cus = rt().info.compute_units;
+  const int64_t fill[2] = {cus * std::max(occ, 1), cus * std::max(occ_large, 1)};
+  int64_t items[2] = {0, 0};  // work items of the two persistent kernels
+  for (auto& g : probs) items[g.cfg == CFG_LARGE] += (int64_t)g.tiles_m * g.tiles_n * std::max(g.splitk, 1);
+  for (;;) {
+    int best = -1;
+    double best_cost = 0;
+    for (size_t i = 0; i < probs.size(); ++i) {
+      const GProb& g = probs[i];
+      const int q = g.cfg == CFG_LARGE;
+      if (!dflt[i] || items[q] >= fill[q] || g.splitk >= 64 || g.k / (2 * g.splitk) < 256) continue;
+      const double bm = CFG_BM[g.cfg];
+      const double cost = bm * bm * (double)g.k / g.splitk;
+      if (cost > best_cost) {
+        best_cost = cost;
+        best = (int)i;
+      }
+    }
+    if (best < 0) break;
+    GProb& g = probs[best];
+    items[g.cfg == CFG_LARGE] += (int64_t)g.tiles_m * g.tiles_n * g.splitk;
+    g.splitk *= 2;
+  }
+  // k per split: a multiple of 32, and no split comes out empty
+  for (auto& g : probs) {
     int64_t kc = (g.k + g.splitk - 1) / std::max(g.splitk, 1);
     kc = std::max<int64_t>(32, (kc + 31) / 32 * 32);
-    int sk = (int)std::max<int64_t>(1, (g.k + kc - 1) / kc);
-    if (g.k == 0) sk = 1;
-    g.splitk = sk;
-    g.kchunk = std::max<int64_t>(kc, 32);
+    g.splitk = g.k == 0 ? 1 : (int)std::max<int64_t>(1, (g.k + kc - 1) / kc);
+    g.kchunk = kc;
+  }
+  return JB_OK;
+}
+
+int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& forced, GroupedPlanImpl& P) {
+  P.method = method;
+  P.kind = kind_of(method);
+  const size_t elt = P.kind == 0 ? 8 : 4;
+  hipError_t e;
+  int occ = 0, occ_large = 0;
+  if ((e = grouped_occupancy(P.kind, &occ)) != hipSuccess) return hip_fail(e, "grouped plan: occupancy query");
+  if (P.kind == 0 && (e = grouped_occupancy(3, &occ_large)) != hipSuccess)
+    return hip_fail(e, "grouped plan: occupancy query");
+  std::vector<GProb> probs;
+  int rc = plan_choices(method, count, d, forced, probs, &P.flops);
+  if (rc) return rc;
+  // pass 3: arrival counters and partial slabs of the split problems
+  size_t part_elems = 0;
+  int counters = 0;
+  std::vector<size_t> part_off(probs.size(), (size_t)-1);
+  std::vector<double> prob_cost(probs.size());
+  size_t nall = 0;
+  for (size_t pi = 0; pi < probs.size(); ++pi) {
+    GProb& g = probs[pi];
+    const int bm = CFG_BM[g.cfg];
     const int ntiles = g.tiles_m * g.tiles_n;
     g.counter_base = counters;
-    if (sk > 1) {
+    if (g.splitk > 1) {
       counters += ntiles;
-      part_off.push_back(part_elems);
-      part_elems += (size_t)ntiles * sk * bm * bm;
-    } else {
-      part_off.push_back((size_t)-1);
+      part_off[pi] = part_elems;
+      part_elems += (size_t)ntiles * g.splitk * bm * bm;
     }
-    const double tile_cost = (double)bm * bm * (double)std::max<int64_t>(g.kchunk, 32) + 4096.0;
-    for (int t = 0; t < ntiles; ++t)
-      for (int s = 0; s < sk; ++s) work.push_back({tile_cost, GItem{(int)probs.size(), t, s, 0}});
-    P.flops += 2.0 * (double)g.m * (double)g.n * (double)x.k;
-    probs.push_back(g);
+    prob_cost[pi] = item_cost(g) * ntiles * g.splitk;
+    nall += (size_t)ntiles * g.splitk;
   }
-  // longest work first
-  std::stable_sort(work.begin(), work.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  if (nall >= ((size_t)1 << 31) || (size_t)counters >= ((size_t)1 << 30))
+    return fail(JB_ERR_UNSUPPORTED, "grouped plan: %zu work items is too many for one launch", nall);
+  // pass 4: work queues.  Launch c < 4 runs the 128x128 items of transpose
+  // combination c, launch 4 the other items; each launch has one queue per
+  // XCD.  A problem with at least 2 NXCD items is cut into NXCD runs of
+  // consecutive items, one per queue: items are ordered split by split, and
+  // within a split tile by tile in groups of `gr` tile-rows (column by column
+  // inside a group), so a run is a compact block of tiles sharing A row
+  // panels and B column panels.  A smaller problem goes whole to the queue of
+  // its launch with the least work so far (problems visited in decreasing
+  // order of their work).  Each queue runs its longest items first.
+  std::vector<std::pair<double, GItem>> q[NLAUNCH][NXCD];
+  double load[NLAUNCH][NXCD] = {};
+  std::vector<int> order(probs.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return prob_cost[x] > prob_cost[y]; });
+  std::vector<GItem> seq;
+  for (int pi : order) {
+    const GProb& g = probs[pi];
+    const int L = g.cfg == CFG_LARGE ? 2 * g.ta + g.tb : NLAUNCH - 1;
+    const int ntiles = g.tiles_m * g.tiles_n;
+    const int nitems = ntiles * g.splitk;
+    const double c = item_cost(g);
+    const int per_run = (nitems + NXCD - 1) / NXCD;
+    const int gr = std::max(1, std::min(g.tiles_m, (int)std::lround(std::sqrt((double)per_run))));
+    seq.clear();
+    for (int s2 = 0; s2 < g.splitk; ++s2)
+      for (int r0 = 0; r0 < g.tiles_m; r0 += gr)
+        for (int tn = 0; tn < g.tiles_n; ++tn)
+          for (int tm = r0; tm < std::min(r0 + gr, g.tiles_m); ++tm) seq.push_back(GItem{pi, tm * g.tiles_n + tn, s2, 0});
+    if (nitems >= 2 * NXCD) {
+      for (int i = 0; i < nitems; ++i) {
+        const int x = (int)((int64_t)i * NXCD / nitems);
+        q[L][x].push_back({c, seq[i]});
+        load[L][x] += c;
+      }
+    } else {
+      int x = 0;
+      for (int y = 1; y < NXCD; ++y)
+        if (load[L][y] < load[L][x]) x = y;
+      for (const GItem& w : seq) q[L][x].push_back({c, w});
+      load[L][x] += c * nitems;
+    }
+  }
   P.nprobs = (int)probs.size();
-  if (work.size() >= ((size_t)1 << 31) || (size_t)counters >= ((size_t)1 << 31))
-    return fail(JB_ERR_UNSUPPORTED, "grouped plan: %zu work items is too many for one launch", work.size());
-  P.nitems = (int)work.size();
-  P.ncounters = counters + 1;
+  P.nitems = (int)nall;
+  P.ncounters = NLAUNCH * NXCD + counters;  // queue counters, then per-tile arrival counters
   const size_t ob_probs = 0;
   const size_t ob_items = al256(ob_probs + probs.size() * sizeof(GProb));
-  const size_t ob_cnt = al256(ob_items + work.size() * sizeof(GItem));
+  const size_t ob_qdesc = al256(ob_items + nall * sizeof(GItem));
+  const size_t ob_cnt = al256(ob_qdesc + (size_t)NLAUNCH * 2 * NXCD * sizeof(int));
   const size_t ob_part = al256(ob_cnt + (size_t)P.ncounters * sizeof(int));
   const size_t total = al256(ob_part + part_elems * elt) + 256;
   const char* fault = getenv("JB_FAULT_ALLOC");
   if (fault && *fault == '1') return fail(JB_ERR_ALLOC, "grouped plan: allocation refused (JB_FAULT_ALLOC)");
-  hipError_t e = hipMalloc(&P.dev, total);
+  e = hipMalloc(&P.dev, total);
   if (e != hipSuccess) {
     P.dev = nullptr;
     hipGetLastError();
@@ -833,38 +1245,62 @@ int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& f
   char* base = static_cast<char*>(P.dev);
   P.probs = reinterpret_cast<GProb*>(base + ob_probs);
   P.items = reinterpret_cast<GItem*>(base + ob_items);
+  P.qdesc = reinterpret_cast<int*>(base + ob_qdesc);
   P.next = reinterpret_cast<int*>(base + ob_cnt);
   for (size_t i = 0; i < probs.size(); ++i)
     probs[i].part = part_off[i] == (size_t)-1 ? nullptr : base + ob_part + part_off[i] * elt;
-  std::vector<GItem> items(work.size());
-  for (size_t i = 0; i < work.size(); ++i) {
-    items[i] = work[i].second;
-    items[i].prob = work[i].second.prob;
+  std::vector<GItem> items;
+  std::vector<int> qdesc((size_t)NLAUNCH * 2 * NXCD, 0);
+  items.reserve(nall);
+  auto by_cost = [](const std::pair<double, GItem>& x, const std::pair<double, GItem>& y) { return x.first > y.first; };
+  const int cus = rt().info.compute_units;
+  for (int L = 0; L < NLAUNCH; ++L) {
+    P.litems[L] = 0;
+    for (int x = 0; x < NXCD; ++x) {
+      std::stable_sort(q[L][x].begin(), q[L][x].end(), by_cost);
+      qdesc[(size_t)L * 2 * NXCD + x] = (int)items.size();
+      qdesc[(size_t)L * 2 * NXCD + NXCD + x] = (int)q[L][x].size();
+      for (auto& w : q[L][x]) items.push_back(w.second);
+      P.litems[L] += (int)q[L][x].size();
+    }
+    const int occ_l = L < NLAUNCH - 1 ? std::max(occ_large, 1) : occ;
+    P.lgrid[L] = std::max(1, std::min(P.litems[L], cus * occ_l));
   }
-  // counters live after next[0]
-  for (auto& g : probs) g.counter_base += 0;
   if (!probs.empty() && (e = hipMemcpy(P.probs, probs.data(), probs.size() * sizeof(GProb), hipMemcpyHostToDevice)) != hipSuccess)
     return hip_fail(e, "grouped plan: upload problems");
   if (!items.empty() && (e = hipMemcpy(P.items, items.data(), items.size() * sizeof(GItem), hipMemcpyHostToDevice)) != hipSuccess)
     return hip_fail(e, "grouped plan: upload work list");
-  int occ = 0;
-  if ((e = grouped_occupancy(P.kind, &occ)) != hipSuccess) return hip_fail(e, "grouped plan: occupancy query");
-  P.grid = std::max(1, std::min(P.nitems, rt().info.compute_units * occ));
+  if ((e = hipMemcpy(P.qdesc, qdesc.data(), qdesc.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_fail(e, "grouped plan: upload queue descriptors");
   return JB_OK;
 }
 
-// Enqueues [reset counters, persistent kernel] on s.
+// Enqueues [reset counters, 128x128 persistent kernels, persistent kernel] on s.
 hipError_t plan_enqueue(const GroupedPlanImpl& P, hipStream_t s) {
   if (P.nitems == 0) return hipSuccess;
   hipLaunchKernelGGL(zero_i32_kernel, dim3((P.ncounters + 255) / 256 < 1024 ? (P.ncounters + 255) / 256 : 1024),
                      dim3(256), 0, s, P.next, P.ncounters);
   timing_begin(s, "grouped_gemm");
-  if (P.kind == 0)
-    hipLaunchKernelGGL(grouped_kernel<0>, dim3(P.grid), dim3(GNT), 0, s, P.probs, P.items, P.nitems, P.next, P.next + 1);
-  else if (P.kind == 1)
-    hipLaunchKernelGGL(grouped_kernel<1>, dim3(P.grid), dim3(GNT), 0, s, P.probs, P.items, P.nitems, P.next, P.next + 1);
-  else
-    hipLaunchKernelGGL(grouped_kernel<2>, dim3(P.grid), dim3(GNT), 0, s, P.probs, P.items, P.nitems, P.next, P.next + 1);
+  int* const tiles = P.next + NLAUNCH * NXCD;
+  for (int L = 0; L < NLAUNCH; ++L) {
+    if (P.litems[L] == 0) continue;
+    const int* qd = P.qdesc + L * 2 * NXCD;
+    int* qn = P.next + L * NXCD;
+    const dim3 grid(P.lgrid[L]), block(GNT);
+    if (L < NLAUNCH - 1) {
+      auto kern = L == 0 ? grouped_large_kernel<false, false>
+                  : L == 1 ? grouped_large_kernel<false, true>
+                  : L == 2 ? grouped_large_kernel<true, false>
+                           : grouped_large_kernel<true, true>;
+      hipLaunchKernelGGL(kern, grid, block, 0, s, P.probs, P.items, qd, qn, tiles);
+    } else if (P.kind == 0) {
+      hipLaunchKernelGGL(grouped_kernel<0>, grid, block, 0, s, P.probs, P.items, qd, qn, tiles);
+    } else if (P.kind == 1) {
+      hipLaunchKernelGGL(grouped_kernel<1>, grid, block, 0, s, P.probs, P.items, qd, qn, tiles);
+    } else {
+      hipLaunchKernelGGL(grouped_kernel<2>, grid, block, 0, s, P.probs, P.items, qd, qn, tiles);
+    }
+  }
   timing_end(s, "grouped_gemm");
   return hipGetLastError();
 }
@@ -1136,9 +1572,10 @@ int jb_autotune_grouped(int method, int64_t count, const jb_gemm_desc* descs, in
     const int64_t kmax = buckets[key].k;
     TuneEntry best{-1, 1, cur};
     for (int cfg = 0; cfg < NCFG && rc == JB_OK; ++cfg)
-      for (int sk = 1; sk <= 8 && rc == JB_OK; sk *= 2) {
+      for (int sk = 1; sk <= 16 && rc == JB_OK; sk *= 2) {
         if (sk > 1 && kmax / sk < 128) continue;
         if (cfg == CFG_SMALL && (sk > 1 || kmax > SMALL_MAX)) continue;  // one split; k must fit
+        if (cfg == CFG_LARGE && kind_of(method) != 0) continue;             // FP64 only
         over[key] = TuneEntry{cfg, sk, 0.0};
         double t = 0;
         rc = autotune_trial(method, count, descs, over, reps, s, e0, e1, &t);
@@ -1184,6 +1621,32 @@ int jb_autotune_lookup(int method, int64_t m, int64_t n, int64_t k, int* tile, i
   if (!tile || !splitk) return fail(-5, "output pointer is NULL");
   std::lock_guard<std::mutex> lk(rt().mu);
   choose(method, m, n, k, *tile, *splitk);
+  return JB_OK;
+}
+
+int jb_grouped_choices(int method, int64_t count, const jb_gemm_desc* descs, int* tile, int* splitk,
+                       int64_t* kchunk) {
+  int rc = validate_descs(method, count, descs);
+  if (rc) return rc;
+  if (count > 0 && (!tile || !splitk || !kchunk)) return fail(-5, "output pointer is NULL");
+  std::lock_guard<std::mutex> lk(rt().mu);
+  if ((rc = ensure_init())) return rc;
+  std::vector<GProb> probs;
+  double flops = 0;
+  if ((rc = plan_choices(method, count, descs, Forced(), probs, &flops))) return rc;
+  size_t j = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    if (descs[i].m == 0 || descs[i].n == 0) {
+      tile[i] = -1;
+      splitk[i] = 0;
+      kchunk[i] = 0;
+      continue;
+    }
+    tile[i] = probs[j].cfg;
+    splitk[i] = probs[j].splitk;
+    kchunk[i] = probs[j].kchunk;
+    ++j;
+  }
   return JB_OK;
 }
 

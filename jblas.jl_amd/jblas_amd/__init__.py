@@ -9,8 +9,8 @@ from ._lib import (JB_F32_BF16X3, JB_F32_NATIVE, JB_F64_NATIVE, JB_F64_OZAKI_BAS
                    device_info, finalize, init, kernel_time, kernel_timing, kernel_timing_reset, last_error, lib)
 from .api import (DimensionMismatch, device_matrix, fill_normal_, fill_uniform_, fill_wide_, gemm_, initkernel_, jmul_, fastmul_,
                   kernel_, nonfinite_check)
-from .grouped import (GemmDesc, GroupedPlan, autotune, gemm_grouped_, split_chunk, tune_load, tune_lookup,
-                      tune_reset, tune_save)
+from .grouped import (GemmDesc, GroupedPlan, autotune, gemm_grouped_, grouped_choices, split_chunk, tune_load,
+                      tune_lookup, tune_reset, tune_save)
 
 __all__ = [
     "JBError", "DimensionMismatch", "lib", "init", "finalize", "last_error", "device_info",
@@ -19,7 +19,7 @@ __all__ = [
     "device_matrix",
     "nonfinite_check",
     "GemmDesc", "GroupedPlan", "gemm_grouped_", "autotune", "tune_save", "tune_load", "tune_reset",
-    "tune_lookup", "split_chunk",
+    "tune_lookup", "split_chunk", "grouped_choices",
     "JB_F64_NATIVE", "JB_F64_RECURRENCE", "JB_F32_NATIVE", "JB_F32_BF16X3", "JB_F64_OZAKI_BASE",
 ]
 

@@ -104,6 +104,7 @@ SIGNATURES = {
     "jb_autotune_load": (_int, [C.c_char_p]),
     "jb_autotune_lookup": (_int, [_int, _i64, _i64, _i64, C.POINTER(_int), C.POINTER(_int)]),
     "jb_autotune_reset": (_int, []),
+    "jb_grouped_choices": (_int, [_int, _i64, C.c_void_p, C.POINTER(_int), C.POINTER(_int), C.POINTER(_i64)]),
     "jb_gemm_pullback_dev": (_int, [_int, _ch, _ch, _i64, _i64, _i64, _dbl, _p, _i64, _p, _i64, _p, _i64, _p, _i64,
                                     _p, _i64, _p]),
     "jb_gemm_pullback": (_int, [_int, _ch, _ch, _i64, _i64, _i64, _dbl, _p, _i64, _p, _i64, _p, _i64, _p, _i64,

@@ -18,9 +18,9 @@ from . import _lib
 from .api import DimensionMismatch, _Operand, _stream_of, _trans
 
 METHODS = {"f64": _lib.JB_F64_NATIVE, "f32": _lib.JB_F32_NATIVE, "f32_bf16x3": _lib.JB_F32_BF16X3}
-# work-item configurations: MFMA tiles, or the whole problem on the vector ALUs
-# (m, n, k <= 64, the fastmul! form)
-TILES = {0: "64x64", 1: "32x32", 2: "small"}
+# work-item configurations: MFMA tiles, the whole problem on the vector ALUs
+# (m, n, k <= 64, the fastmul! form), the 128x128 FP64 tile (own persistent kernel)
+TILES = {0: "64x64", 1: "32x32", 2: "small", 3: "128x128"}
 
 
 def method_id(method) -> int:
@@ -125,7 +125,9 @@ def tune_reset() -> None:
 
 
 def tune_lookup(method, m: int, n: int, k: int):
-    """(tile name, split-K) the library uses for this shape."""
+    """(tile name, split-K) the table or the static rule gives this shape on its
+    own.  A plan may raise the split further to fill the device (grouped_choices
+    reports the plan's final choice)."""
     t, s = C.c_int(), C.c_int()
     _lib.call("jb_autotune_lookup", method_id(method), m, n, k, C.byref(t), C.byref(s))
     return TILES[t.value], s.value
@@ -135,3 +137,15 @@ def split_chunk(k: int, splitk: int) -> int:
     """k per split used by the grouped launch (a multiple of 32, never an empty split)."""
     kc = -(-k // max(splitk, 1))
     return max(32, -(-kc // 32) * 32)
+
+
+def grouped_choices(method, descs: Sequence[GemmDesc]):
+    """Per problem, the (tile name, split-K, k per split) a plan of this group
+    would use now (None for a problem with m or n == 0).  Split s sums
+    k-values [s*kchunk, min(k, (s+1)*kchunk)); the splits' partial results are
+    added in split order."""
+    arr = _desc_array(descs)
+    cnt = max(1, len(descs))
+    t, s, kc = (C.c_int * cnt)(), (C.c_int * cnt)(), (C.c_int64 * cnt)()
+    _lib.call("jb_grouped_choices", method_id(method), len(descs), C.cast(arr, C.c_void_p), t, s, kc)
+    return [None if t[i] < 0 else (TILES[t[i]], s[i], kc[i]) for i in range(len(descs))]

@@ -159,10 +159,12 @@ def test_no_device_is_a_clean_error():
 
 
 def test_grouped_static_choice_rule():
-    """The planner's static choice (no tuning table; no GPU needed): problems
-    with m, n, k <= 64 run whole in one work item (fastmul!'s form,
-    src/kernels.jl:202-208); larger outputs take the 64x64 MFMA tile from
-    max(m, n) >= 96, else 32x32; a small output with a long k splits k."""
+    """The planner's static choice per problem (no tuning table; no GPU
+    needed): problems with m, n, k <= 64 run whole in one work item (fastmul!'s
+    form, src/kernels.jl:202-208); FP64 outputs with m, n >= 128 that fill at
+    least 85 % of their 128x128 tiles take the 128x128 tile; other outputs the
+    64x64 tile from max(m, n) >= 96, else 32x32; one split (a plan adds splits
+    to fill the device)."""
     from jblas_amd import grouped as G
     G.tune_reset()
     for m, n, k in [(1, 1, 1), (64, 64, 64), (16, 40, 8), (64, 1, 64), (5, 7, 0)]:
@@ -171,5 +173,11 @@ def test_grouped_static_choice_rule():
     assert G.tune_lookup("f64", 65, 64, 64) == ("32x32", 1)
     assert G.tune_lookup("f64", 64, 64, 65) == ("32x32", 1)
     assert G.tune_lookup("f64", 96, 32, 64) == ("64x64", 1)
-    assert G.tune_lookup("f64", 64, 64, 4096) == ("32x32", 1)  # 4 tiles: no split
-    assert G.tune_lookup("f64", 16, 16, 4096) == ("32x32", 4)  # 1 tile: split while k / (2 s) >= 256 and tiles * s < 4
+    assert G.tune_lookup("f64", 64, 64, 4096) == ("32x32", 1)
+    assert G.tune_lookup("f64", 16, 16, 4096) == ("32x32", 1)
+    assert G.tune_lookup("f64", 1024, 1024, 1024) == ("128x128", 1)
+    assert G.tune_lookup("f64", 128, 2000, 8) == ("128x128", 1)   # 2000 of 2048 columns: 97.7 %
+    assert G.tune_lookup("f64", 136, 1024, 64) == ("64x64", 1)    # 136 of 256 rows
+    assert G.tune_lookup("f64", 120, 4096, 64) == ("64x64", 1)    # m < 128
+    assert G.tune_lookup("f32", 1024, 1024, 1024) == ("64x64", 1)  # the 128x128 tile is FP64 only
+    assert G.tune_lookup("f32_bf16x3", 1024, 1024, 1024) == ("64x64", 1)

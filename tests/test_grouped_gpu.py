@@ -4,8 +4,9 @@ autotuner and the gemm! pullback (rrule), through the C ABI.
 Bars:
   * FP64 grouped: BIT-EXACT against the oracle's split-K restatement of the
     reference recurrence (oracle.gemm_recurrence_chunked with the k-chunk the
-    library chose for that problem) -- each split is an fma chain in k order
-    (f64 MFMA) and the partials are summed in split order.
+    plan chose for that problem, as grouped_choices reports it) -- each split
+    is an fma chain in k order (f64 MFMA) and the partials are summed in split
+    order.
   * FP32 (native MFMA and 3xBF16): |C - C*| <= 2*K*2^-24 (|alpha||A||B| + |beta||C0|)
     componentwise against the FP64 oracle.
   * graph replays are bit-identical to the eager launch, replay after replay.
@@ -45,6 +46,24 @@ def dev_copy(torch, a):
     return t.T
 
 
+def kchunks(method, descs):
+    """k per split of every problem in the grouped launch of `descs` (the plan's choice now)."""
+    return [c[2] if c is not None else 32 for c in jb.grouped_choices(method, descs)]
+
+
+def write_table(path, method, shapes, cfg, splitk):
+    """A tuning table (the library's JSON format) with one entry (cfg, splitk) for
+    the size bucket of each shape: (floor(log2(floor(sqrt(m n)))), floor(log2 k))."""
+    info = jb.device_info()
+    pre = f"{info['name']}|{info['arch']}|m{jb.grouped.method_id(method)}|"
+    keys = sorted({f"mn{max(math.isqrt(m * n), 1).bit_length() - 1}|k{max(k, 1).bit_length() - 1}"
+                   for m, n, k in shapes})
+    entries = ",\n".join(f'    {{"key": "{pre}{k}", "cfg": {cfg}, "splitk": {splitk}, "ms": 0.0}}' for k in keys)
+    with open(path, "w") as f:
+        f.write('{\n  "format": "jblas-tune-1",\n  "entries": [\n' + entries + "\n  ]\n}\n")
+    jb.tune_load(path)
+
+
 def make_group(torch, count, seed, dtype, ops=("N", "N"), alpha=1.0, beta=0.0):
     shapes = G.loguniform_shapes(seed, count)
     probs = []
@@ -63,11 +82,13 @@ def test_grouped_f64_c4_bitexact(gpu):
     torch = gpu
     probs, descs = make_group(torch, 1024, 52, torch.float64)
     C0 = [to_host(p[7]) for p in probs[::8]]
+    choices = jb.grouped_choices("f64", descs)
+    assert {c[0] for c in choices} == {"small", "32x32", "64x64", "128x128"}, "every tile kind in the set"
     jb.gemm_grouped_("f64", descs)
     torch.cuda.synchronize()
     for idx, (ta, tb, m, n, k, A, B, C) in enumerate(probs[::8]):
-        tile, sk = jb.tune_lookup("f64", m, n, k)
-        ref = O.gemm_recurrence_chunked(ta, tb, 1.0, to_host(A), to_host(B), 0.0, C0[idx], jb.split_chunk(k, sk))
+        tile, sk, kc = choices[8 * idx]
+        ref = O.gemm_recurrence_chunked(ta, tb, 1.0, to_host(A), to_host(B), 0.0, C0[idx], kc)
         assert np.array_equal(to_host(C), ref), (m, n, k, tile, sk)
 
 
@@ -77,12 +98,12 @@ def test_grouped_f64_ops_alpha_beta(gpu, ta, tb):
     probs, descs = make_group(torch, 48, 60 + ord(ta) + 2 * ord(tb), torch.float64, ops=(ta, tb), alpha=1.5,
                               beta=0.5)
     C0 = [to_host(p[7]) for p in probs]
+    kc = kchunks("f64", descs)
     jb.gemm_grouped_("f64", descs)
     torch.cuda.synchronize()
     for idx, (ta_, tb_, m, n, k, A, B, C) in enumerate(probs):
-        _, sk = jb.tune_lookup("f64", m, n, k)
-        ref = O.gemm_recurrence_chunked(ta, tb, 1.5, to_host(A), to_host(B), 0.5, C0[idx], jb.split_chunk(k, sk))
-        assert np.array_equal(to_host(C), ref), (m, n, k, sk)
+        ref = O.gemm_recurrence_chunked(ta, tb, 1.5, to_host(A), to_host(B), 0.5, C0[idx], kc[idx])
+        assert np.array_equal(to_host(C), ref), (m, n, k, kc[idx])
 
 
 def test_grouped_forced_splitk_bitexact(gpu):
@@ -96,17 +117,70 @@ def test_grouped_forced_splitk_bitexact(gpu):
         C = dev_uniform(torch, m, n, 30 + i, torch.float64)
         probs.append((m, n, k, A, B, C))
     C0 = [to_host(p[5]) for p in probs]
-    choices = [jb.tune_lookup("f64", m, n, k) for (m, n, k, *_r) in probs]
-    assert any(sk > 1 for _, sk in choices), choices
+    descs = [GemmDesc("N", "N", 1.0, A, B, 1.0, C) for (m, n, k, A, B, C) in probs]
+    choices = jb.grouped_choices("f64", descs)
+    # statically one split each; the plan splits k while the group has fewer
+    # work items than the device holds resident workgroups, down to 256 k a split
+    assert all(jb.tune_lookup("f64", m, n, k)[1] == 1 for (m, n, k, *_r) in probs)
+    assert all(sk > 1 and kc >= 256 for _, sk, kc in choices), choices
     for rep in range(2):  # the combine must give the same bits every time
         for i, p in enumerate(probs):
             p[5].copy_(to_dev(torch, C0[i]))
-        jb.gemm_grouped_("f64", [GemmDesc("N", "N", 1.0, A, B, 1.0, C) for (m, n, k, A, B, C) in probs])
+        jb.gemm_grouped_("f64", descs)
         torch.cuda.synchronize()
         for i, (m, n, k, A, B, C) in enumerate(probs):
-            ref = O.gemm_recurrence_chunked("N", "N", 1.0, to_host(A), to_host(B), 1.0, C0[i],
-                                            jb.split_chunk(k, choices[i][1]))
+            ref = O.gemm_recurrence_chunked("N", "N", 1.0, to_host(A), to_host(B), 1.0, C0[i], choices[i][2])
             assert np.array_equal(to_host(C), ref), (m, n, k, choices[i])
+
+
+LARGE_SHAPES = [(256, 256, 160), (200, 130, 77), (128, 384, 48), (131, 129, 1000), (130, 140, 0)]
+
+
+def strided(torch, rows, cols, seed, ld, offset):
+    """rows x cols column-major view with leading dimension `ld` starting
+    `offset` elements into a fresh buffer (odd ld / offset: no 16-byte vectors)."""
+    buf = torch.empty(offset + ld * max(cols, 1), dtype=torch.float64, device="cuda")
+    jb.fill_uniform_(buf, seed)
+    return buf[offset:offset + ld * cols].view(cols, ld).T[:rows] if cols else buf[:0].view(0, rows).T
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("splitk", [1, 3])
+def test_grouped_large_tile_bitexact(gpu, ta, tb, splitk, tmp_path):
+    """The 128x128 FP64 tile (its own persistent kernel per transpose
+    combination), forced through a tuning table on fringe shapes (partial tiles
+    in m, n and k; k == 0), with 16-byte vector loads (aligned bases, even
+    leading dimensions) and without (odd leading dimension or an odd element
+    offset), one split or three: bit-exact against the chunked recurrence."""
+    torch = gpu
+    jb.tune_reset()
+    write_table(str(tmp_path / "large.json"), "f64", LARGE_SHAPES, 3, splitk)
+    try:
+        probs = []
+        for i, (m, n, k) in enumerate(LARGE_SHAPES):
+            ra, ca = (m, k) if ta == "N" else (k, m)
+            rb, cb = (k, n) if tb == "N" else (n, k)
+            odd = i % 2 == 1  # every other problem on the element-wise loads
+            A = strided(torch, ra, ca, 300 + 3 * i, ra + (1 if odd else 2), 1 if odd else 0)
+            B = strided(torch, rb, cb, 301 + 3 * i, rb + (3 if odd else 0), 0)
+            C = strided(torch, m, n, 302 + 3 * i, m + 5, 3)
+            probs.append((m, n, k, A, B, C))
+        C0 = [to_host(p[5]) for p in probs]
+        for alpha, beta in [(1.5, 0.5), (-1.0, 0.0)]:
+            for i, p in enumerate(probs):
+                p[5].copy_(to_dev(torch, C0[i]))
+            descs = [GemmDesc(ta, tb, alpha, A, B, beta, C) for (m, n, k, A, B, C) in probs]
+            choices = jb.grouped_choices("f64", descs)
+            assert all(c[0] == "128x128" for c in choices), choices
+            jb.gemm_grouped_("f64", descs)
+            torch.cuda.synchronize()
+            for i, (m, n, k, A, B, C) in enumerate(probs):
+                ref = O.gemm_recurrence_chunked(ta, tb, alpha, to_host(A), to_host(B), beta, C0[i], choices[i][2])
+                got = to_host(C)
+                assert np.array_equal(got.view(np.int64), ref.view(np.int64)), (ta, tb, m, n, k, choices[i], alpha,
+                                                                               beta)
+    finally:
+        jb.tune_reset()
 
 
 SMALL_SHAPES = [(1, 1, 1), (64, 64, 64), (33, 17, 5), (7, 64, 63), (64, 3, 1), (50, 40, 64)]
@@ -124,19 +198,10 @@ def small_group(torch, ta, tb, dtype, seed):
 
 
 def load_small_table(tmpdir, method, shapes):
-    """A tuning table (the library's JSON format) naming the small path for the
-    size buckets of `shapes`: bucket (floor(log2(floor(sqrt(m n)))), floor(log2 k)).
+    """A tuning table naming the small path for the size buckets of `shapes`.
     (The static rule picks it for these shapes too; the table also covers the
     bucket of a problem that does not fit it.)"""
-    info = jb.device_info()
-    pre = f"{info['name']}|{info['arch']}|m{jb.grouped.method_id(method)}|"
-    keys = sorted({f"mn{max(math.isqrt(m * n), 1).bit_length() - 1}|k{max(k, 1).bit_length() - 1}"
-                   for m, n, k in shapes})
-    path = os.path.join(tmpdir, "small.json")
-    entries = ",\n".join(f'    {{"key": "{pre}{k}", "cfg": 2, "splitk": 1, "ms": 0.0}}' for k in keys)
-    with open(path, "w") as f:
-        f.write('{\n  "format": "jblas-tune-1",\n  "entries": [\n' + entries + "\n  ]\n}\n")
-    jb.tune_load(path)
+    write_table(os.path.join(tmpdir, "small.json"), method, shapes, 2, 1)
 
 
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
@@ -151,18 +216,19 @@ def test_grouped_small_path_f64_bitexact(gpu, ta, tb, tmp_path):
     load_small_table(str(tmp_path), "f64", SMALL_SHAPES)
     assert jb.tune_lookup("f64", 65, 63, 64)[0] == "32x32"  # bucket of (50, 40, 64), m too large
     probs = small_group(torch, ta, tb, torch.float64, 70 + ord(ta) + 2 * ord(tb))
-    kinds = [jb.tune_lookup("f64", m, n, k) for (m, n, k, *_r) in probs]
-    assert [t for t, _ in kinds[:len(SMALL_SHAPES)]] == ["small"] * len(SMALL_SHAPES)
-    assert all(t != "small" for t, _ in kinds[len(SMALL_SHAPES):]), kinds
     C0 = [to_host(p[5]) for p in probs]
     for alpha, beta in [(1.5, 0.5), (-1.0, 0.0), (0.0, 2.0)]:
         for i, p in enumerate(probs):
             p[5].copy_(to_dev(torch, C0[i]))
-        jb.gemm_grouped_("f64", [GemmDesc(ta, tb, alpha, A, B, beta, C) for (m, n, k, A, B, C) in probs])
+        descs = [GemmDesc(ta, tb, alpha, A, B, beta, C) for (m, n, k, A, B, C) in probs]
+        kinds = jb.grouped_choices("f64", descs)
+        assert [c[0] for c in kinds[:len(SMALL_SHAPES)]] == ["small"] * len(SMALL_SHAPES)
+        if alpha != 0.0:  # alpha == 0 means no products (k taken as 0): every problem fits the small path
+            assert all(c[0] != "small" for c in kinds[len(SMALL_SHAPES):]), kinds
+        jb.gemm_grouped_("f64", descs)
         torch.cuda.synchronize()
         for i, (m, n, k, A, B, C) in enumerate(probs):
-            ref = O.gemm_recurrence_chunked(ta, tb, alpha, to_host(A), to_host(B), beta, C0[i],
-                                            jb.split_chunk(k, kinds[i][1]))
+            ref = O.gemm_recurrence_chunked(ta, tb, alpha, to_host(A), to_host(B), beta, C0[i], kinds[i][2])
             assert np.array_equal(to_host(C), ref), (ta, tb, m, n, k, kinds[i], alpha, beta)
     jb.tune_reset()
 
@@ -205,12 +271,13 @@ def test_grouped_f64_bits_with_signed_zeros(gpu):
         B = np.asfortranarray(rng.choice([-1.0, -0.0, 0.0, 3.0], (k, n), p=[.2, .3, .3, .2]))
         C = np.asfortranarray(rng.choice([-0.0, 0.0, 2.0], (m, n), p=[.4, .4, .2]))
         probs.append((m, n, k, alpha, beta, A, B, C, dev_copy(torch, A), dev_copy(torch, B), dev_copy(torch, C)))
-    jb.gemm_grouped_("f64", [GemmDesc("N", "N", p[3], p[8], p[9], p[4], p[10]) for p in probs])
+    descs = [GemmDesc("N", "N", p[3], p[8], p[9], p[4], p[10]) for p in probs]
+    kc = kchunks("f64", descs)
+    jb.gemm_grouped_("f64", descs)
     torch.cuda.synchronize()
-    for (m, n, k, alpha, beta, A, B, C, Ad, Bd, Cd) in probs:
-        _, sk = jb.tune_lookup("f64", m, n, k)
-        ref = O.gemm_recurrence_chunked("N", "N", alpha, A, B, beta, C, jb.split_chunk(k, sk))
-        assert np.array_equal(to_host(Cd).view(np.int64), ref.view(np.int64)), (m, n, k, alpha, beta, sk)
+    for i, (m, n, k, alpha, beta, A, B, C, Ad, Bd, Cd) in enumerate(probs):
+        ref = O.gemm_recurrence_chunked("N", "N", alpha, A, B, beta, C, kc[i])
+        assert np.array_equal(to_host(Cd).view(np.int64), ref.view(np.int64)), (m, n, k, alpha, beta, kc[i])
 
 
 def test_grouped_plan_graph_replays(gpu):
@@ -325,12 +392,11 @@ def test_autotune_roundtrip(gpu):
             jb.tune_load(path)
     # the tuned table still gives bit-exact results
     C0 = [to_host(p[7]) for p in probs]
+    kc = kchunks("f64", descs)
     jb.gemm_grouped_("f64", descs)
     torch.cuda.synchronize()
     for idx, (ta, tb, m, n, k, A, B, C) in enumerate(probs[::4]):
-        _, sk = jb.tune_lookup("f64", m, n, k)
-        ref = O.gemm_recurrence_chunked("N", "N", 1.0, to_host(A), to_host(B), 0.0, C0[4 * idx],
-                                        jb.split_chunk(k, sk))
+        ref = O.gemm_recurrence_chunked("N", "N", 1.0, to_host(A), to_host(B), 0.0, C0[4 * idx], kc[4 * idx])
         assert np.array_equal(to_host(C), ref)
     jb.tune_reset()
 
@@ -346,11 +412,11 @@ def test_autotune_leaves_c_untouched(gpu):
     torch.cuda.synchronize()
     for p, c0 in zip(probs, C0):
         assert np.array_equal(to_host(p[7]), c0)
+    kc = kchunks("f64", descs)
     jb.gemm_grouped_("f64", descs)
     torch.cuda.synchronize()
     for idx, (ta, tb, m, n, k, A, B, C) in enumerate(probs):
-        _, sk = jb.tune_lookup("f64", m, n, k)
-        ref = O.gemm_recurrence_chunked("N", "N", 1.5, to_host(A), to_host(B), 0.5, C0[idx], jb.split_chunk(k, sk))
+        ref = O.gemm_recurrence_chunked("N", "N", 1.5, to_host(A), to_host(B), 0.5, C0[idx], kc[idx])
         assert np.array_equal(to_host(C), ref)
     jb.tune_reset()
 

@@ -7,8 +7,8 @@ After each call every canary word must be unchanged (compared bitwise), for
 fringe shapes that leave partial tiles in m, n and k, on every kernel family:
 the FP64/FP32 MFMA GEMM (each tile variant), the sequential-fma kernels,
 kernel!/initkernel!, the split-BF16 / split-INT8 / 3xBF16 slice-pair kernels
-(including the allocation-free form) and the grouped persistent kernel
-(MFMA tiles, split-K slabs and the small-problem path).
+(including the allocation-free form) and the grouped persistent kernels
+(MFMA tiles up to 128x128, split-K slabs and the small-problem path).
 """
 import ctypes
 
@@ -112,7 +112,8 @@ def test_noalloc_guard_bands(gpu):
 @pytest.mark.parametrize("method,dtype", [("f64", np.float64), ("f32", np.float32), ("f32_bf16x3", np.float32)])
 def test_grouped_guard_bands(gpu, method, dtype, tmp_path):
     """Mixed shapes in one launch: small-path problems (m, n, k <= 64), MFMA
-    tiles with fringes and forced split-K through a tuning table."""
+    tiles with fringes, and tiles and split-K forced through a tuning table
+    (the 128x128 FP64 tile included)."""
     torch = gpu
     shapes = [(17, 9, 5), (64, 64, 64), (130, 70, 45), (33, 200, 700), (1, 1, 1), (96, 40, 1500)]
     guards, descs = [], []
@@ -127,21 +128,26 @@ def test_grouped_guard_bands(gpu, method, dtype, tmp_path):
     jb.gemm_grouped_(method, descs)
     for i, (g, _, _) in enumerate(guards):
         g.check(f"grouped {method} problem {i} {shapes[i]}")
-    # the same group with split-K forced on the long-k buckets (tuning table)
+    # the same group with tile and split-K forced on every bucket (tuning
+    # table): 32x32 tiles with 4 splits; for FP64 also the 128x128 tile with
+    # one split and with 2
     path = tmp_path / "tune.json"
     dev = jb.device_info()
     mid = jb.grouped.method_id(method)
     # the table's bucket key: method, floor(log2(floor(sqrt(m n)))), floor(log2 k)
     keys = [f"m{mid}|mn{int(np.sqrt(float(m) * n)).bit_length() - 1}|k{k.bit_length() - 1}" for (m, n, k) in shapes]
-    entries = ",\n".join(f'    {{"key": "{dev["name"]}|{dev["arch"]}|{key}", "cfg": 1, "splitk": 4, "ms": 1.0}}'
-                         for key in keys)
-    path.write_text('{\n  "format": "jblas-tune-1",\n  "entries": [\n' + entries + "\n  ]\n}\n")
-    for g, _, _ in guards:
-        g.before = g.buf.cpu().numpy().view(g.itype).copy()
-    jb.tune_load(str(path))
+    forced = [(1, 4)] + ([(3, 1), (3, 2)] if method == "f64" else [])
     try:
-        jb.gemm_grouped_(method, descs)
-        for i, (g, _, _) in enumerate(guards):
-            g.check(f"grouped {method} split-K problem {i} {shapes[i]}")
+        for cfg, splitk in forced:
+            entries = ",\n".join(f'    {{"key": "{dev["name"]}|{dev["arch"]}|{key}", "cfg": {cfg}, '
+                                 f'"splitk": {splitk}, "ms": 1.0}}' for key in keys)
+            path.write_text('{\n  "format": "jblas-tune-1",\n  "entries": [\n' + entries + "\n  ]\n}\n")
+            for g, _, _ in guards:
+                g.before = g.buf.cpu().numpy().view(g.itype).copy()
+            jb.tune_reset()
+            jb.tune_load(str(path))
+            jb.gemm_grouped_(method, descs)
+            for i, (g, _, _) in enumerate(guards):
+                g.check(f"grouped {method} tile {cfg} split-K {splitk} problem {i} {shapes[i]}")
     finally:
         jb.tune_reset()
