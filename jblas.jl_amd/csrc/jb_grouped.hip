@@ -38,23 +38,22 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
 constexpr int GNT = 256;  // 4 waves
-
-// Operand and result pointers of a problem come from its descriptor in memory,
-// so the compiler cannot infer their address space: without this cast every
-// access is a flat access, which also counts against the LDS wait counter
-// (an LDS wait then waits for the operand prefetch too).  Global-space
-// pointers give global loads and stores (and scalar-base addressing).
-template <typename T> using gptr = __attribute__((address_space(1))) T*;
-template <typename T> __device__ __forceinline__ gptr<T> as_global(T* p) { return (gptr<T>)p; }
+// (Operand and result pointers come from problem descriptors in memory: every
+// access goes through as_global, jb_internal.h.)
 // Work-item configurations: 0 = 64x64 MFMA tile, 1 = 32x32 MFMA tile, 2 = the
-// whole problem on the vector ALUs (m, n, k <= SMALL_MAX, one split), 3 =
-// 128x128 MFMA tile (FP64; run by a second persistent kernel whose register
-// budget is the large tile's, so the others keep four workgroups per CU).
-constexpr int NCFG = 4;
-constexpr int CFG_BM[NCFG] = {64, 32, 64, 128};
+// whole problem on the vector ALUs (m, n, k <= SMALL_MAX, one split); FP64
+// only, run by a second persistent kernel whose register budget is the
+// 128x128 tile's (so the others keep four workgroups per CU): 3 = 128x128,
+// 4 = 128x64, 5 = 64x128 MFMA tiles.
+constexpr int NCFG = 6;
+constexpr int CFG_BM[NCFG] = {64, 32, 64, 128, 128, 64};  // tile rows
+constexpr int CFG_BN[NCFG] = {64, 32, 64, 128, 64, 128};  // tile columns
 constexpr int CFG_SMALL = 2;
 constexpr int CFG_LARGE = 3;
+constexpr int CFG_TALL = 4;
+constexpr int CFG_WIDE = 5;
 constexpr int SMALL_MAX = 64;
+__host__ __device__ constexpr bool large_cfg(int cfg) { return cfg >= CFG_LARGE; }
 
 // Device descriptor of one problem.
 struct GProb {
@@ -382,34 +381,36 @@ __device__ void native_tile(const GProb& P, int tm, int tn, int split, char* sme
   splitk_combine<T, BM, BN>(P, tm, tn, counters, sflag);
 }
 
-// 128x128 FP64 tile: four waves of 64x64 (4x4 blocks of v_mfma_f64_16x16x4f64),
-// 16 k per LDS stage, two LDS buffers, the next stage's loads one stage ahead
-// in registers.  Same chain semantics as native_tile: chains start from -0.0,
-// A's padding is -0.0, each k-split is one fma chain in increasing k.
-template <bool TA, bool TB>
+// BM x BN FP64 tile (128x128, 128x64 or 64x128): four waves of (BM/2)x(BN/2)
+// (blocks of v_mfma_f64_16x16x4f64), 16 k per LDS stage, two LDS buffers,
+// the next stage's loads one stage ahead in registers.  Same chain semantics
+// as native_tile: chains start from -0.0, A's padding is -0.0, each k-split
+// is one fma chain in increasing k.
+template <int BM, int BN, bool TA, bool TB>
 __device__ void large_tile_f64(const GProb& P, int tm, int tn, int split, char* smem, int* counters, int* sflag) {
-  constexpr int BM = 128, BK = 16, ST = BM + 16;  // row stride 144: lane groups g, g+1 start 32 banks apart
-  constexpr int IM = 4, JN = 4;
+  // LDS row strides extent + 16 doubles: lane groups g, g+1 of a fragment read start 32 banks apart
+  constexpr int BK = 16, STA = BM + 16, STB = BN + 16;
+  constexpr int IM = BM / 32, JN = BN / 32;
   constexpr bool SWA = TA, SWB = !TB;  // k-contiguous operands are stored swizzled
   double* const As = reinterpret_cast<double*>(smem);
-  double* const Bs = As + 2 * BK * ST;
+  double* const Bs = As + 2 * BK * STA;
   const int tid = work_item_tid();
   const int lane = tid % 64, wave = tid / 64;
-  const int wm0 = (wave % 2) * 64, wn0 = (wave / 2) * 64;
+  const int wm0 = (wave % 2) * (BM / 2), wn0 = (wave / 2) * (BN / 2);
   const int g = lane / 16, l16 = lane % 16;
-  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BM;
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
   const int64_t kb = (int64_t)split * P.kchunk;
   const int64_t ke = min(P.k, kb + P.kchunk);
   gptr<const double> A = as_global(static_cast<const double*>(P.A));
   gptr<const double> B = as_global(static_cast<const double*>(P.B));
   GVecLoader<BM, !TA, true> la(tid);
-  GVecLoader<BM, TB, false> lb(tid);
+  GVecLoader<BN, TB, false> lb(tid);
   // 16-byte loads: aligned bases and even leading dimensions (k0 and the
   // vectors' start rows are even)
   bool va, ea, vb, eb;
   vec_conditions<!TA>(P.A, P.lda, P.m, ke, va, ea);
   vec_conditions<TB>(P.B, P.ldb, P.n, ke, vb, eb);
-  const bool fa = m0 + BM <= P.m, fb = n0 + BM <= P.n;
+  const bool fa = m0 + BM <= P.m, fb = n0 + BN <= P.n;
   d4 acc[IM][JN];
 #pragma unroll
   for (int i = 0; i < IM; ++i)
@@ -429,8 +430,8 @@ __device__ void large_tile_f64(const GProb& P, int tm, int tn, int split, char* 
   };
   __syncthreads();  // LDS may still be read by the previous work item
   load_stage(0);
-  la.template store<ST, SWA>(As);
-  lb.template store<ST, SWB>(Bs);
+  la.template store<STA, SWA>(As);
+  lb.template store<STB, SWB>(Bs);
   if (nkt > 1) load_stage(1);
   __syncthreads();
   // Swizzled column of (w0 + 16 i + l16) in k-row 4t + g: 16 ((w0/16 + i) ^ (t >> 1))
@@ -441,15 +442,15 @@ __device__ void large_tile_f64(const GProb& P, int tm, int tn, int split, char* 
     const int swc = (t & 1) ? sw1 : sw0;
 #pragma unroll
     for (int i = 0; i < IM; ++i)
-      a[i] = as[(4 * t + g) * ST + (SWA ? 16 * ((bm0 + i) ^ (t >> 1)) + swc : wm0 + 16 * i + l16)];
+      a[i] = as[(4 * t + g) * STA + (SWA ? 16 * ((bm0 + i) ^ (t >> 1)) + swc : wm0 + 16 * i + l16)];
 #pragma unroll
     for (int j = 0; j < JN; ++j)
-      b[j] = bs[(4 * t + g) * ST + (SWB ? 16 * ((bn0 + j) ^ (t >> 1)) + swc : wn0 + 16 * j + l16)];
+      b[j] = bs[(4 * t + g) * STB + (SWB ? 16 * ((bn0 + j) ^ (t >> 1)) + swc : wn0 + 16 * j + l16)];
   };
   for (int kt = 0; kt < nkt; ++kt) {
     const int cur = kt & 1;
-    const double* as = As + cur * BK * ST;
-    const double* bs = Bs + cur * BK * ST;
+    const double* as = As + cur * BK * STA;
+    const double* bs = Bs + cur * BK * STB;
     double a[2][IM], b[2][JN];
     frags(as, bs, 0, a[0], b[0]);
 #pragma unroll
@@ -461,8 +462,8 @@ __device__ void large_tile_f64(const GProb& P, int tm, int tn, int split, char* 
         for (int j = 0; j < JN; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[t & 1][j], a[t & 1][i], acc[i][j], 0, 0, 0);
       if (t == BK / 4 - 2 && kt + 1 < nkt) {
-        la.template store<ST, SWA>(As + (cur ^ 1) * BK * ST);
-        lb.template store<ST, SWB>(Bs + (cur ^ 1) * BK * ST);
+        la.template store<STA, SWA>(As + (cur ^ 1) * BK * STA);
+        lb.template store<STB, SWB>(Bs + (cur ^ 1) * BK * STB);
         if (kt + 2 < nkt) load_stage(kt + 2);
       }
     }
@@ -492,7 +493,7 @@ __device__ void large_tile_f64(const GProb& P, int tm, int tn, int split, char* 
     }
   } else {
     gptr<double> slab =
-        as_global(static_cast<double*>(P.part)) + ((int64_t)(tm * P.tiles_n + tn) * P.splitk + split) * (BM * BM);
+        as_global(static_cast<double*>(P.part)) + ((int64_t)(tm * P.tiles_n + tn) * P.splitk + split) * (BM * BN);
 #pragma unroll
     for (int i = 0; i < IM; ++i)
 #pragma unroll
@@ -500,7 +501,7 @@ __device__ void large_tile_f64(const GProb& P, int tm, int tn, int split, char* 
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           slab[(wm0 + 16 * i + l16) + (wn0 + 16 * j + GMfma<double>::prow(r, g)) * BM] = acc[i][j][r];
-    splitk_combine<double, BM, BM>(P, tm, tn, counters, sflag);
+    splitk_combine<double, BM, BN>(P, tm, tn, counters, sflag);
   }
 }
 
@@ -852,15 +853,16 @@ __global__ __launch_bounds__(GNT, 2) void grouped_kernel(const GProb* __restrict
   }
 }
 
-// The 128x128 tiles (FP64) of a group with one transpose combination, pulled
-// from their own queue (one kernel per combination: a single kernel holding all
-// four tile variants exceeds the register budget of two workgroups per CU).
-template <bool TA, bool TB>
+// The FP64 items of one 128-row/column tile shape and transpose combination,
+// pulled from their own queues (one kernel per shape and combination: a kernel
+// holding several tile variants exceeds the register budget of two
+// workgroups per CU).
+template <int BM, int BN, bool TA, bool TB>
 __global__ __launch_bounds__(GNT, 2) void grouped_large_kernel(const GProb* __restrict__ probs,
                                                                const GItem* __restrict__ items,
                                                                const int* __restrict__ qdesc, int* __restrict__ qnext,
                                                                int* __restrict__ counters) {
-  constexpr int SMEM = 2 * 2 * 16 * (128 + 16) * 8;  // two buffers of A and B, 16 k-rows of stride 144
+  constexpr int SMEM = 2 * 16 * ((BM + 16) + (BN + 16)) * 8;  // two buffers of A and B: 16 k-rows of stride extent + 16
   __shared__ __attribute__((aligned(16))) char smem[SMEM];
   __shared__ int s_item, s_flag;
   int q = blockIdx.x % NXCD, tries = 0;
@@ -870,7 +872,7 @@ __global__ __launch_bounds__(GNT, 2) void grouped_large_kernel(const GProb* __re
     const GItem w = items[it];
     const GProb P = probs[w.prob];
     const int tm = w.tile / P.tiles_n, tn = w.tile % P.tiles_n;
-    large_tile_f64<TA, TB>(P, tm, tn, w.split, smem, counters, &s_flag);
+    large_tile_f64<BM, BN, TA, TB>(P, tm, tn, w.split, smem, counters, &s_flag);
   }
 }
 
@@ -913,38 +915,46 @@ std::string bucket_key(int method, int64_t m, int64_t n, int64_t k) {
 // Static defaults (no table entry).  Tile: the small path for m, n, k <= 64
 // (one work item, B staged in LDS: on 1024 such problems it measured 43.0 us
 // against 44.4 us for the tiles in FP64 and 34.7 against 37.6 us in 3xBF16);
-// otherwise the largest tile whose padded area is still mostly useful -- for
-// FP64 the 128x128 tile when both m and n reach 128 and at least 85 % of its
-// padded area is C, else 64x64 from max(m, n) >= 96, else 32x32.  Split-K is
-// not chosen per problem: plan_build splits the longest work items when the
-// whole group has fewer work items than the device holds workgroups (the
-// blocking decision that depends on the device, blocking_structure's role in
-// src/memory_management.jl:78-140).
+// otherwise, for FP64, the tile with the least padded area per unit of
+// measured throughput (TILE_RATE: each tile's FP64 rate on groups of
+// 1024 x 1024 problems with k = 256 and 1024, relative to the 128x128 tile),
+// and for FP32 the 64x64 tile from
+// max(m, n) >= 96, else 32x32.  Split-K is not chosen per problem: plan_build
+// splits the longest work items when the whole group has fewer work items than
+// the device holds workgroups (the blocking decision that depends on the
+// device, blocking_structure's role in src/memory_management.jl:78-140).
+constexpr double TILE_RATE[NCFG] = {0.90, 0.65, 0.0, 1.0, 1.0, 0.97};
 void default_choice(int kind, int64_t m, int64_t n, int64_t k, int& cfg, int& splitk) {
   splitk = 1;
   if (m <= SMALL_MAX && n <= SMALL_MAX && k <= SMALL_MAX) {
     cfg = CFG_SMALL;
     return;
   }
-  if (kind == 0 && m >= 128 && n >= 128) {
-    const double padded = (double)((m + 127) / 128 * 128) * (double)((n + 127) / 128 * 128);
-    if ((double)m * (double)n >= 0.85 * padded) {
-      cfg = CFG_LARGE;
-      return;
+  if (kind != 0) {
+    cfg = std::max(m, n) >= 96 ? 0 : 1;
+    return;
+  }
+  double best = 0;
+  for (int c : {CFG_LARGE, CFG_TALL, CFG_WIDE, 0, 1}) {
+    const double padded = (double)((m + CFG_BM[c] - 1) / CFG_BM[c] * CFG_BM[c]) *
+                          (double)((n + CFG_BN[c] - 1) / CFG_BN[c] * CFG_BN[c]);
+    const double t = padded / TILE_RATE[c];
+    if (c == CFG_LARGE || t < best) {
+      best = t;
+      cfg = c;
     }
   }
-  cfg = std::max(m, n) >= 96 ? 0 : 1;
 }
 
 // A choice taken from a bucket (table, tuner trial, forced) may name the small
 // path for a problem that does not fit it: such a problem takes the 32x32 tile;
-// the 128x128 tile exists for FP64 only (others take 64x64).
+// the 128-row/column tiles exist for FP64 only (others take 64x64).
 void fit_choice(int kind, int64_t m, int64_t n, int64_t k, int& cfg, int& splitk) {
   if (cfg == CFG_SMALL) {
     if (m <= SMALL_MAX && n <= SMALL_MAX && k <= SMALL_MAX) splitk = 1;
     else cfg = 1;
   }
-  if (cfg == CFG_LARGE && kind != 0) cfg = 0;
+  if (large_cfg(cfg) && kind != 0) cfg = 0;
 }
 
 // Returns true when the choice is the static default (plan_build may then add split-K).
@@ -966,13 +976,15 @@ bool choose(int method, int64_t m, int64_t n, int64_t k, int& cfg, int& splitk) 
 // ---------------------------------------------------------------------------
 // plans
 // ---------------------------------------------------------------------------
-constexpr int NLAUNCH = 5;  // persistent launches of a plan: the 128x128 kernel per transpose combination, the rest
+// persistent launches of a plan: L = 4 (cfg - CFG_LARGE) + 2 ta + tb for the
+// 128-row/column FP64 tiles (one kernel per tile shape and transpose
+// combination), then the kernel of the other tiles
+constexpr int NLAUNCH = 13;
 
 // Relative duration of one work item of problem g (MFMA work of one k-split
 // of one tile plus a fixed epilogue share).
 double item_cost(const GProb& g) {
-  const double bm = CFG_BM[g.cfg];
-  return bm * bm * (double)std::max<int64_t>(g.kchunk, 32) + 4096.0;
+  return (double)CFG_BM[g.cfg] * CFG_BN[g.cfg] * (double)std::max<int64_t>(g.kchunk, 32) + 4096.0;
 }
 
 // forced choice for the autotuner's trial launches: cfg >= 0 forces one choice
@@ -987,14 +999,20 @@ struct Forced {
 struct GroupedPlanImpl {
   int method = 0, kind = 0;
   int nitems = 0, nprobs = 0, ncounters = 0;
-  // launches 0-3: the 128x128 kernel of transpose combination c; 4: the other items
-  int litems[NLAUNCH] = {0, 0, 0, 0, 0}, lgrid[NLAUNCH] = {0, 0, 0, 0, 0};
+  // items and grid of each launch (see NLAUNCH)
+  int litems[NLAUNCH] = {}, lgrid[NLAUNCH] = {};
   void* dev = nullptr;  // probs | items | queue descriptors | counters | partials
   size_t dev_bytes = 0;
   GProb* probs = nullptr;
   GItem* items = nullptr;
   int* qdesc = nullptr;  // per launch: NXCD queue offsets into items, then NXCD queue lengths
   int* next = nullptr;   // per launch NXCD queue counters, then per-tile arrival counters (zeroed every launch)
+  // the persistent launches run concurrently: all but the first on side
+  // streams forked from the caller's stream after the counter reset and
+  // joined back to it (a launch's tail overlaps the others' work)
+  int nside = 0;
+  hipStream_t side[NLAUNCH - 1] = {};
+  hipEvent_t fork = nullptr, join[NLAUNCH - 1] = {};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t cap = nullptr;
@@ -1048,7 +1066,7 @@ hipError_t grouped_occupancy(int kind, int* occ) {
     hipError_t e = kind == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<0>, GNT, 0)
                    : kind == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<1>, GNT, 0)
                    : kind == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<2>, GNT, 0)
-                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_large_kernel<true, true>, GNT, 0);
+                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_large_kernel<128, 128, true, true>, GNT, 0);
     if (e != hipSuccess) return e;
     cached[kind] = std::min(std::max(n, 1), 8);
   }
@@ -1109,9 +1127,8 @@ int plan_choices(int method, int64_t count, const jb_gemm_desc* d, const Forced&
       }
     }
     fit_choice(kind, g.m, g.n, g.k, g.cfg, g.splitk);
-    const int bm = CFG_BM[g.cfg];
-    g.tiles_m = (int)((g.m + bm - 1) / bm);
-    g.tiles_n = (int)((g.n + bm - 1) / bm);
+    g.tiles_m = (int)((g.m + CFG_BM[g.cfg] - 1) / CFG_BM[g.cfg]);
+    g.tiles_n = (int)((g.n + CFG_BN[g.cfg] - 1) / CFG_BN[g.cfg]);
     *flops += 2.0 * (double)g.m * (double)g.n * (double)x.k;
     probs.push_back(g);
     dflt.push_back(is_default && g.cfg != CFG_SMALL);
@@ -1119,16 +1136,15 @@ int plan_choices(int method, int64_t count, const jb_gemm_desc* d, const Forced&
   const int64_t cus = rt().info.compute_units;
   const int64_t fill[2] = {cus * std::max(occ, 1), cus * std::max(occ_large, 1)};
   int64_t items[2] = {0, 0};  // work items of the two persistent kernels
-  for (auto& g : probs) items[g.cfg == CFG_LARGE] += (int64_t)g.tiles_m * g.tiles_n * std::max(g.splitk, 1);
+  for (auto& g : probs) items[large_cfg(g.cfg)] += (int64_t)g.tiles_m * g.tiles_n * std::max(g.splitk, 1);
   for (;;) {
     int best = -1;
     double best_cost = 0;
     for (size_t i = 0; i < probs.size(); ++i) {
       const GProb& g = probs[i];
-      const int q = g.cfg == CFG_LARGE;
+      const int q = large_cfg(g.cfg);
       if (!dflt[i] || items[q] >= fill[q] || g.splitk >= 64 || g.k / (2 * g.splitk) < 256) continue;
-      const double bm = CFG_BM[g.cfg];
-      const double cost = bm * bm * (double)g.k / g.splitk;
+      const double cost = (double)CFG_BM[g.cfg] * CFG_BN[g.cfg] * (double)g.k / g.splitk;
       if (cost > best_cost) {
         best_cost = cost;
         best = (int)i;
@@ -1136,7 +1152,7 @@ int plan_choices(int method, int64_t count, const jb_gemm_desc* d, const Forced&
     }
     if (best < 0) break;
     GProb& g = probs[best];
-    items[g.cfg == CFG_LARGE] += (int64_t)g.tiles_m * g.tiles_n * g.splitk;
+    items[large_cfg(g.cfg)] += (int64_t)g.tiles_m * g.tiles_n * g.splitk;
     g.splitk *= 2;
   }
   // k per split: a multiple of 32, and no split comes out empty
@@ -1169,22 +1185,21 @@ int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& f
   size_t nall = 0;
   for (size_t pi = 0; pi < probs.size(); ++pi) {
     GProb& g = probs[pi];
-    const int bm = CFG_BM[g.cfg];
     const int ntiles = g.tiles_m * g.tiles_n;
     g.counter_base = counters;
     if (g.splitk > 1) {
       counters += ntiles;
       part_off[pi] = part_elems;
-      part_elems += (size_t)ntiles * g.splitk * bm * bm;
+      part_elems += (size_t)ntiles * g.splitk * CFG_BM[g.cfg] * CFG_BN[g.cfg];
     }
     prob_cost[pi] = item_cost(g) * ntiles * g.splitk;
     nall += (size_t)ntiles * g.splitk;
   }
   if (nall >= ((size_t)1 << 31) || (size_t)counters >= ((size_t)1 << 30))
     return fail(JB_ERR_UNSUPPORTED, "grouped plan: %zu work items is too many for one launch", nall);
-  // pass 4: work queues.  Launch c < 4 runs the 128x128 items of transpose
-  // combination c, launch 4 the other items; each launch has one queue per
-  // XCD.  A problem with at least 2 NXCD items is cut into NXCD runs of
+  // pass 4: work queues.  One launch per 128-row/column tile shape and
+  // transpose combination, then one for the other items (NLAUNCH); each
+  // launch has one queue per XCD.  A problem with at least 2 NXCD items is cut into NXCD runs of
   // consecutive items, one per queue: items are ordered split by split, and
   // within a split tile by tile in groups of `gr` tile-rows (column by column
   // inside a group), so a run is a compact block of tiles sharing A row
@@ -1199,7 +1214,7 @@ int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& f
   std::vector<GItem> seq;
   for (int pi : order) {
     const GProb& g = probs[pi];
-    const int L = g.cfg == CFG_LARGE ? 2 * g.ta + g.tb : NLAUNCH - 1;
+    const int L = large_cfg(g.cfg) ? 4 * (g.cfg - CFG_LARGE) + 2 * g.ta + g.tb : NLAUNCH - 1;
     const int ntiles = g.tiles_m * g.tiles_n;
     const int nitems = ntiles * g.splitk;
     const double c = item_cost(g);
@@ -1272,40 +1287,79 @@ int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& f
     return hip_fail(e, "grouped plan: upload work list");
   if ((e = hipMemcpy(P.qdesc, qdesc.data(), qdesc.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
     return hip_fail(e, "grouped plan: upload queue descriptors");
+  int nlaunch = 0;
+  for (int L = 0; L < NLAUNCH; ++L) nlaunch += P.litems[L] > 0;
+  if (nlaunch > 1) {
+    if ((e = hipEventCreateWithFlags(&P.fork, hipEventDisableTiming)) != hipSuccess)
+      return hip_fail(e, "grouped plan: event");
+    for (; P.nside < nlaunch - 1; ++P.nside) {
+      if ((e = hipStreamCreateWithFlags(&P.side[P.nside], hipStreamNonBlocking)) != hipSuccess)
+        return hip_fail(e, "grouped plan: side stream");
+      if ((e = hipEventCreateWithFlags(&P.join[P.nside], hipEventDisableTiming)) != hipSuccess) {
+        hipStreamDestroy(P.side[P.nside]);
+        P.side[P.nside] = nullptr;
+        return hip_fail(e, "grouped plan: event");
+      }
+    }
+  }
   return JB_OK;
 }
 
-// Enqueues [reset counters, 128x128 persistent kernels, persistent kernel] on s.
+typedef void (*GroupedKernelFn)(const GProb*, const GItem*, const int*, int*, int*);
+const GroupedKernelFn kLargeKernels[NLAUNCH - 1] = {
+    grouped_large_kernel<128, 128, false, false>, grouped_large_kernel<128, 128, false, true>,
+    grouped_large_kernel<128, 128, true, false>,  grouped_large_kernel<128, 128, true, true>,
+    grouped_large_kernel<128, 64, false, false>,  grouped_large_kernel<128, 64, false, true>,
+    grouped_large_kernel<128, 64, true, false>,   grouped_large_kernel<128, 64, true, true>,
+    grouped_large_kernel<64, 128, false, false>,  grouped_large_kernel<64, 128, false, true>,
+    grouped_large_kernel<64, 128, true, false>,   grouped_large_kernel<64, 128, true, true>};
+
+// Enqueues [reset counters, 128-row/column persistent kernels, persistent kernel] on s.
 hipError_t plan_enqueue(const GroupedPlanImpl& P, hipStream_t s) {
   if (P.nitems == 0) return hipSuccess;
   hipLaunchKernelGGL(zero_i32_kernel, dim3((P.ncounters + 255) / 256 < 1024 ? (P.ncounters + 255) / 256 : 1024),
                      dim3(256), 0, s, P.next, P.ncounters);
   timing_begin(s, "grouped_gemm");
+  hipError_t e = hipSuccess;
+  if (P.nside > 0 && (e = hipEventRecord(P.fork, s)) != hipSuccess) return e;
   int* const tiles = P.next + NLAUNCH * NXCD;
+  int j = -1;  // launch j + 1 > 0 goes to side stream j
   for (int L = 0; L < NLAUNCH; ++L) {
     if (P.litems[L] == 0) continue;
+    hipStream_t ls = s;
+    if (j >= 0) {
+      ls = P.side[j];
+      if ((e = hipStreamWaitEvent(ls, P.fork, 0)) != hipSuccess) return e;
+    }
     const int* qd = P.qdesc + L * 2 * NXCD;
     int* qn = P.next + L * NXCD;
     const dim3 grid(P.lgrid[L]), block(GNT);
     if (L < NLAUNCH - 1) {
-      auto kern = L == 0 ? grouped_large_kernel<false, false>
-                  : L == 1 ? grouped_large_kernel<false, true>
-                  : L == 2 ? grouped_large_kernel<true, false>
-                           : grouped_large_kernel<true, true>;
-      hipLaunchKernelGGL(kern, grid, block, 0, s, P.probs, P.items, qd, qn, tiles);
+      hipLaunchKernelGGL(kLargeKernels[L], grid, block, 0, ls, P.probs, P.items, qd, qn, tiles);
     } else if (P.kind == 0) {
-      hipLaunchKernelGGL(grouped_kernel<0>, grid, block, 0, s, P.probs, P.items, qd, qn, tiles);
+      hipLaunchKernelGGL(grouped_kernel<0>, grid, block, 0, ls, P.probs, P.items, qd, qn, tiles);
     } else if (P.kind == 1) {
-      hipLaunchKernelGGL(grouped_kernel<1>, grid, block, 0, s, P.probs, P.items, qd, qn, tiles);
+      hipLaunchKernelGGL(grouped_kernel<1>, grid, block, 0, ls, P.probs, P.items, qd, qn, tiles);
     } else {
-      hipLaunchKernelGGL(grouped_kernel<2>, grid, block, 0, s, P.probs, P.items, qd, qn, tiles);
+      hipLaunchKernelGGL(grouped_kernel<2>, grid, block, 0, ls, P.probs, P.items, qd, qn, tiles);
     }
+    if (j >= 0) {
+      if ((e = hipEventRecord(P.join[j], ls)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(s, P.join[j], 0)) != hipSuccess) return e;
+    }
+    ++j;
   }
   timing_end(s, "grouped_gemm");
   return hipGetLastError();
 }
 
 void plan_free(GroupedPlanImpl& P) {
+  for (int i = 0; i < P.nside; ++i) {
+    if (P.side[i]) hipStreamSynchronize(P.side[i]);
+    if (P.side[i]) hipStreamDestroy(P.side[i]);
+    if (P.join[i]) hipEventDestroy(P.join[i]);
+  }
+  if (P.fork) hipEventDestroy(P.fork);
   if (P.exec) hipGraphExecDestroy(P.exec);
   if (P.graph) hipGraphDestroy(P.graph);
   if (P.cap) hipStreamDestroy(P.cap);
@@ -1575,7 +1629,7 @@ int jb_autotune_grouped(int method, int64_t count, const jb_gemm_desc* descs, in
       for (int sk = 1; sk <= 16 && rc == JB_OK; sk *= 2) {
         if (sk > 1 && kmax / sk < 128) continue;
         if (cfg == CFG_SMALL && (sk > 1 || kmax > SMALL_MAX)) continue;  // one split; k must fit
-        if (cfg == CFG_LARGE && kind_of(method) != 0) continue;             // FP64 only
+        if (large_cfg(cfg) && kind_of(method) != 0) continue;               // FP64 only
         over[key] = TuneEntry{cfg, sk, 0.0};
         double t = 0;
         rc = autotune_trial(method, count, descs, over, reps, s, e0, e1, &t);

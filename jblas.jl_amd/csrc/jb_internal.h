@@ -80,6 +80,14 @@ int runtime_compute_units();
 void timing_begin(hipStream_t s, const char* kernel);
 void timing_end(hipStream_t s, const char* kernel);
 
+// Pointers that a kernel reads from memory (a descriptor, a struct argument)
+// carry no address space the compiler can infer, so accesses through them are
+// flat accesses, which also count against the LDS wait counter (a wait for an
+// LDS read then waits for outstanding global traffic too).  Casting them to
+// the global address space gives global loads and stores.
+template <typename T> using gptr = __attribute__((address_space(1))) T*;
+template <typename T> __device__ __forceinline__ gptr<T> as_global(T* p) { return (gptr<T>)p; }
+
 // XCD-aware block swizzle: hardware places workgroup w on XCD (w % nxcd).
 // Returns a tile id such that each XCD owns a contiguous range of tile ids.
 __device__ __forceinline__ int xcd_remap(int w, int nblocks, int nxcd) {

@@ -675,17 +675,18 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   }
 
   if (MODE != 1) {
-    double* C = static_cast<double*>(eC);
+    gptr<double> C = as_global(static_cast<double*>(eC));
+    gptr<const int> eAg = as_global(p.eA), eBg = as_global(p.eB);
 #pragma unroll
     for (int b = 0; b < WB; ++b) {
       const int64_t m = m0 + wm0 + 32 * b + l32;
       if (m >= p.M) continue;
-      const int ea = p.eA[m];
+      const int ea = eAg[m];
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
         const int64_t n = n0 + wn0 + 8 * (e / 4) + 4 * hl + (e % 4);
         if (n < eN) {
-          const int eb = p.eB[n];
+          const int eb = eBg[n];
           double hi = 0.0, lo = 0.0;
 #pragma unroll
           for (int g = NG - 1; g >= 0; --g) {
@@ -702,7 +703,7 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
       }
     }
   } else {
-    float* C = static_cast<float*>(eC);
+    gptr<float> C = as_global(static_cast<float*>(eC));
     const float alpha = (float)ealpha, beta = (float)ebeta;
 #pragma unroll
     for (int b = 0; b < WB; ++b) {

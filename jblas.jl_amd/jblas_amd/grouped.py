@@ -19,8 +19,9 @@ from .api import DimensionMismatch, _Operand, _stream_of, _trans
 
 METHODS = {"f64": _lib.JB_F64_NATIVE, "f32": _lib.JB_F32_NATIVE, "f32_bf16x3": _lib.JB_F32_BF16X3}
 # work-item configurations: MFMA tiles, the whole problem on the vector ALUs
-# (m, n, k <= 64, the fastmul! form), the 128x128 FP64 tile (own persistent kernel)
-TILES = {0: "64x64", 1: "32x32", 2: "small", 3: "128x128"}
+# (m, n, k <= 64, the fastmul! form), and the FP64 tiles with 128 rows or
+# columns (their own persistent kernels)
+TILES = {0: "64x64", 1: "32x32", 2: "small", 3: "128x128", 4: "128x64", 5: "64x128"}
 
 
 def method_id(method) -> int:

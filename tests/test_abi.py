@@ -161,8 +161,9 @@ def test_no_device_is_a_clean_error():
 def test_grouped_static_choice_rule():
     """The planner's static choice per problem (no tuning table; no GPU
     needed): problems with m, n, k <= 64 run whole in one work item (fastmul!'s
-    form, src/kernels.jl:202-208); FP64 outputs with m, n >= 128 that fill at
-    least 85 % of their 128x128 tiles take the 128x128 tile; other outputs the
+    form, src/kernels.jl:202-208); other FP64 problems take the tile with the
+    least padded area divided by the tile's measured relative rate (128x128
+    and 128x64: 1, 64x128: 0.97, 64x64: 0.9, 32x32: 0.65); FP32 problems the
     64x64 tile from max(m, n) >= 96, else 32x32; one split (a plan adds splits
     to fill the device)."""
     from jblas_amd import grouped as G
@@ -170,14 +171,20 @@ def test_grouped_static_choice_rule():
     for m, n, k in [(1, 1, 1), (64, 64, 64), (16, 40, 8), (64, 1, 64), (5, 7, 0)]:
         assert G.tune_lookup("f64", m, n, k) == ("small", 1), (m, n, k)
         assert G.tune_lookup("f32_bf16x3", m, n, k) == ("small", 1), (m, n, k)
-    assert G.tune_lookup("f64", 65, 64, 64) == ("32x32", 1)
-    assert G.tune_lookup("f64", 64, 64, 65) == ("32x32", 1)
-    assert G.tune_lookup("f64", 96, 32, 64) == ("64x64", 1)
-    assert G.tune_lookup("f64", 64, 64, 4096) == ("32x32", 1)
-    assert G.tune_lookup("f64", 16, 16, 4096) == ("32x32", 1)
-    assert G.tune_lookup("f64", 1024, 1024, 1024) == ("128x128", 1)
-    assert G.tune_lookup("f64", 128, 2000, 8) == ("128x128", 1)   # 2000 of 2048 columns: 97.7 %
-    assert G.tune_lookup("f64", 136, 1024, 64) == ("64x64", 1)    # 136 of 256 rows
-    assert G.tune_lookup("f64", 120, 4096, 64) == ("64x64", 1)    # m < 128
-    assert G.tune_lookup("f32", 1024, 1024, 1024) == ("64x64", 1)  # the 128x128 tile is FP64 only
-    assert G.tune_lookup("f32_bf16x3", 1024, 1024, 1024) == ("64x64", 1)
+    rate = {"128x128": 1.0, "128x64": 1.0, "64x128": 0.97, "64x64": 0.90, "32x32": 0.65}
+
+    def rule(m, n):  # first minimum in the order listed
+        def padded(t):
+            bm, bn = map(int, t.split("x"))
+            return -(-m // bm) * bm * (-(-n // bn) * bn) / rate[t]
+        return min(rate, key=padded)
+
+    for (m, n, k, want) in [(1024, 1024, 1024, "128x128"), (1000, 200, 64, "128x128"), (1024, 64, 512, "128x64"),
+                            (64, 1024, 512, "64x128"), (65, 64, 64, "128x64"), (96, 32, 64, "32x32"),
+                            (64, 64, 4096, "64x64"), (16, 16, 4096, "32x32"), (200, 72, 96, "128x128"),
+                            (1000, 136, 300, "128x64"), (136, 1000, 300, "64x128")]:
+        assert rule(m, n) == want, (m, n)
+        assert G.tune_lookup("f64", m, n, k) == (want, 1), (m, n, k)
+    assert G.tune_lookup("f32", 1024, 1024, 1024) == ("64x64", 1)  # the 128-row/column tiles are FP64 only
+    assert G.tune_lookup("f32_bf16x3", 1024, 64, 512) == ("64x64", 1)
+    assert G.tune_lookup("f32_bf16x3", 65, 64, 64) == ("32x32", 1)
