@@ -296,7 +296,9 @@ int jb_grouped_plan_destroy(jb_grouped_plan* plan);
  * every trial times the whole group (fastest of `reps` launches), buckets are
  * visited once in decreasing order of their flops (those under 0.5 % of the
  * total keep the static rule), and a candidate is kept only if it makes the
- * group >= 1 % faster.  Improved buckets go into the in-memory table (replaces
+ * group >= 1 % faster; the tuned choices are then kept only if the group,
+ * timed again alternately with the static rule, is still >= 1 % faster.
+ * Improved buckets go into the in-memory table (replaces
  * the static pick_kernel_size / blocking_structure choice).  The trials write
  * to scratch copies of the outputs: the group's C matrices are neither read
  * nor written.  Synchronous. */

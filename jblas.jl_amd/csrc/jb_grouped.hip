@@ -1289,6 +1289,22 @@ int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& f
     return hip_fail(e, "grouped plan: upload queue descriptors");
   int nlaunch = 0;
   for (int L = 0; L < NLAUNCH; ++L) nlaunch += P.litems[L] > 0;
+  if (log_enabled()) {
+    int per_cfg[NCFG] = {}, split = 0;
+    for (const GProb& g : probs) {
+      ++per_cfg[g.cfg];
+      split += g.splitk > 1;
+    }
+    char launches[256] = "";
+    size_t at = 0;
+    for (int L = 0; L < NLAUNCH && at < sizeof launches; ++L)
+      if (P.litems[L] > 0)
+        at += snprintf(launches + at, sizeof launches - at, " [%d: %d items, grid %d]", L, P.litems[L], P.lgrid[L]);
+    log_line("grouped plan: method %d, %d problems (64x64 %d, 32x32 %d, small %d, 128x128 %d, 128x64 %d, 64x128 %d;"
+             " %d split), %d items, %.4g flops, %zu device bytes, launches%s",
+             method, P.nprobs, per_cfg[0], per_cfg[1], per_cfg[2], per_cfg[3], per_cfg[4], per_cfg[5], split,
+             P.nitems, P.flops, P.dev_bytes, launches);
+  }
   if (nlaunch > 1) {
     if ((e = hipEventCreateWithFlags(&P.fork, hipEventDisableTiming)) != hipSuccess)
       return hip_fail(e, "grouped plan: event");
@@ -1638,9 +1654,28 @@ int jb_autotune_grouped(int method, int64_t count, const jb_gemm_desc* descs, in
     if (best.cfg >= 0) {
       over[key] = best;
       cur = best.ms;
+      log_line("autotune %s: tile %d, split %d, group %.4f ms", key.c_str(), best.cfg, best.splitk, best.ms);
     } else {
       over.erase(key);
     }
+  }
+  // Confirmation: the descent compared single trials, so a choice that only
+  // looked >= 1 % faster through timing noise can slip in; the tuned choices
+  // are kept only if the whole group, timed again alternately with the
+  // static rule (twice as many launches each), is still >= 1 % faster.
+  if (rc == JB_OK && !over.empty()) {
+    const std::map<std::string, TuneEntry> none;
+    double t_static = 1e30, t_tuned = 1e30;
+    for (int round = 0; round < 2 && rc == JB_OK; ++round) {
+      double t = 0;
+      if ((rc = autotune_trial(method, count, descs, none, 2 * reps, s, e0, e1, &t)) != JB_OK) break;
+      t_static = std::min(t_static, t);
+      if ((rc = autotune_trial(method, count, descs, over, 2 * reps, s, e0, e1, &t)) != JB_OK) break;
+      t_tuned = std::min(t_tuned, t);
+    }
+    if (rc == JB_OK) log_line("autotune confirmation: static %.4f ms, tuned %.4f ms: %s", t_static, t_tuned,
+                              t_tuned < 0.99 * t_static ? "kept" : "dropped");
+    if (rc == JB_OK && !(t_tuned < 0.99 * t_static)) over.clear();
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);

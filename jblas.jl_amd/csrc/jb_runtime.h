@@ -34,6 +34,9 @@ struct Runtime {
 
 Runtime& rt();
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+// JB_LOG diagnostics (no-ops unless the variable is set)
+bool log_enabled();
+void log_line(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 int hip_fail(hipError_t e, const char* what);
 int validate_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, bool alpha_zero, const void* A, int64_t lda,
                   const void* B, int64_t ldb, const void* C, int64_t ldc, GemmArgs* out);

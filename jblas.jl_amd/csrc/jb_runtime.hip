@@ -39,6 +39,36 @@ Runtime& rt() {
 
 static thread_local std::string tl_error;
 
+// ---- JB_LOG ------------------------------------------------------------------
+// Quiet unless JB_LOG is set: "1" or "stderr" writes to stderr, any other
+// non-empty value other than "0" names a file that lines are appended to.
+// One line per event ("jblas: " prefix, flushed): errors returned by any entry
+// point, device init, device buffer growth, grouped plans, tuner decisions.
+static std::mutex g_log_mu;
+static FILE* log_stream() {
+  static FILE* f = []() -> FILE* {
+    const char* v = getenv("JB_LOG");
+    if (!v || !*v || strcmp(v, "0") == 0) return nullptr;
+    if (strcmp(v, "1") == 0 || strcmp(v, "stderr") == 0) return stderr;
+    FILE* fp = fopen(v, "a");
+    return fp ? fp : stderr;
+  }();
+  return f;
+}
+bool log_enabled() { return log_stream() != nullptr; }
+void log_line(const char* fmt, ...) {
+  FILE* f = log_stream();
+  if (!f) return;
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  std::lock_guard<std::mutex> lk(g_log_mu);
+  fprintf(f, "jblas: %s\n", buf);
+  fflush(f);
+}
+
 int fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -46,6 +76,7 @@ int fail(int code, const char* fmt, ...) {
   vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
   tl_error = buf;
+  log_line("error %d: %s", code, buf);
   return code;
 }
 
@@ -144,6 +175,8 @@ int init_locked(int device) {
   }
   r.device = device;
   r.inited = true;
+  log_line("init device %d: %s %s, %d CUs, %.2f GHz, LDS %d B/CU, L2 %lld B, HBM %lld B", device, d.name, d.arch,
+           d.compute_units, d.clock_khz * 1e-6, d.lds_bytes_per_cu, (long long)d.l2_bytes, (long long)d.hbm_bytes);
   const char* tf = getenv("JB_TUNE_FILE");
   if (tf && *tf) tune_load_locked(tf);  // a missing or corrupt table leaves the static defaults
   return JB_OK;
@@ -168,6 +201,7 @@ int Buffer::reserve(size_t bytes, const char* what) {
     return fail(JB_ERR_ALLOC, "%s: hipMalloc(%zu) failed: %s", what, want, hipGetErrorString(e));
   }
   size = want;
+  log_line("buffer %s grown to %zu bytes", what, want);
   return JB_OK;
 }
 

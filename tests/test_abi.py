@@ -158,6 +158,39 @@ def test_no_device_is_a_clean_error():
     assert rc == _lib.JB_ERR_HIP and "device" in _lib.last_error()
 
 
+_LOG_PROBE = """
+import ctypes, sys
+sys.path.insert(0, {pkg!r})
+from jblas_amd import _lib
+L = _lib.lib()
+L.jb_dgemm(b"N", b"N", -3, 4, 4, ctypes.c_double(1.0), None, 1, None, 1, ctypes.c_double(0.0), None, 1)
+"""
+
+
+@pytest.mark.parametrize("mode", ["file", "off"])
+def test_jb_log(tmp_path, mode):
+    """JB_LOG: quiet unless set; set to a path, every error an entry point
+    returns is appended there as one line (SURVEY: logging stays quiet unless
+    JB_LOG is set)."""
+    import subprocess
+    import sys
+    log = tmp_path / "jb.log"
+    env = dict(os.environ)
+    env.pop("JB_LOG", None)
+    if mode == "file":
+        env["JB_LOG"] = str(log)
+    code = _LOG_PROBE.format(pkg=os.path.join(ROOT, "jblas.jl_amd"))
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "jblas:" not in r.stderr
+    if mode == "file":
+        lines = log.read_text().splitlines()
+        assert "jblas: error -3: m < 0" in lines, lines  # the third argument (m) is invalid
+        assert all(ln.startswith("jblas: ") for ln in lines), lines
+    else:
+        assert not log.exists()
+
+
 def test_grouped_static_choice_rule():
     """The planner's static choice per problem (no tuning table; no GPU
     needed): problems with m, n, k <= 64 run whole in one work item (fastmul!'s
