@@ -38,8 +38,9 @@ def parse():
     ap.add_argument("--nslices-int8", type=int, default=7, help="INT8 slices (8 bits each)")
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline and the oracle check")
     ap.add_argument("--skip-table", action="store_true", help="omit the per-slice-count table")
-    ap.add_argument("--only", choices=["ozaki", "native", "ozaki_int8", "grouped"], default=None,
-                    help="run only one method, or only the batched and rrule workloads (for profiler runs)")
+    ap.add_argument("--only", choices=["ozaki", "native", "ozaki_int8", "grouped", "grouped_static"], default=None,
+                    help="run only one method, or only the batched and rrule workloads, or only the batched "
+                         "workload's static plans (for profiler runs)")
     ap.add_argument("--skip-grouped", action="store_true",
                     help="omit the batched (1024 uneven problems) and rrule (4096x2048x4096) measurements")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -64,13 +65,13 @@ def timed(torch, fn, steps, warm=False):
     return e0.elapsed_time(e1) / steps
 
 
-def grouped_and_rrule(torch, jb):
+def grouped_and_rrule(torch, jb, tune=True, rrule=True):
     """Secondary workloads, measured after the headline step:
     * batched: 1024 independent problems, m, n, k log-uniform in [16, 2048]
       (multiples of 8), one persistent launch captured in a hipGraph, FP64 and
-      FP32 (3xBF16); static tile choice, then after the autotuner;
+      FP32 (3xBF16); static tile choice, then after the autotuner (tune);
     * rrule: F32 A 4096x2048, B 2048x4096: forward + pullback (dA, dB in one
-      grouped launch)."""
+      grouped launch) (rrule)."""
     import numpy as np
 
     from jblas_amd import generators as G
@@ -93,7 +94,7 @@ def grouped_and_rrule(torch, jb):
             descs.append(GemmDesc("N", "N", 1.0, A.T, B.T, 0.0, C.T))
             keep += [A, B, C]
         res = {}
-        for phase in ("static", "tuned"):
+        for phase in ("static", "tuned") if tune else ("static",):
             if phase == "tuned":
                 t0 = time.perf_counter()
                 jb.autotune(method, descs, reps=2)
@@ -107,7 +108,7 @@ def grouped_and_rrule(torch, jb):
         # the grouped kernels against the matrix-core peak of the arithmetic they
         # run: FP64 MFMA (78.6 TFLOP/s), or BF16 MFMA for 3xBF16 (2500 TFLOP/s;
         # six BF16 products per FP32 product)
-        best = max(res["static_tflops"], res["tuned_tflops"])
+        best = max(res["static_tflops"], res.get("tuned_tflops", 0.0))
         ach, peak = (best, 78.6) if method == "f64" else (6 * best, 2500.0)
         res["roofline"] = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
                            "frac": round(ach / peak, 4),
@@ -120,6 +121,8 @@ def grouped_and_rrule(torch, jb):
         del descs, keep
         torch.cuda.empty_cache()
     out["batched_config"] = "1024 problems, m,n,k log-uniform in [16,2048] (multiples of 8), seed 52; sum 2mnk = %.4g" % flops
+    if not rrule:
+        return out, samples
     # rrule sizes (F32): forward C = A*B and pullback dA = W*B^T, dB = A^T*W
     M, K, N = 4096, 2048, 4096
     A = (torch.randn((K, M), dtype=torch.float32, device="cuda") / 45.25).T  # N(0,1)/sqrt(2048), column-major
@@ -311,8 +314,9 @@ def main():
     log(f"device {info['arch']} CUs={info['compute_units']} clock={info['clock_khz'] / 1e6:.2f}GHz "
         f"peak fp64={info['peak_fp64_tflops']:.1f} bf16={info['peak_bf16_tflops']:.1f} TF")
 
-    if args.only == "grouped":
-        extra, _ = grouped_and_rrule(torch, jb)
+    if args.only in ("grouped", "grouped_static"):
+        full = args.only == "grouped"
+        extra, _ = grouped_and_rrule(torch, jb, tune=full, rrule=full)
         print(json.dumps({"grouped_and_rrule": extra}), flush=True)
         return
 
