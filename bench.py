@@ -125,9 +125,10 @@ def grouped_and_rrule(torch, jb, tune=True, rrule=True):
         return out, samples
     # rrule sizes (F32): forward C = A*B and pullback dA = W*B^T, dB = A^T*W
     M, K, N = 4096, 2048, 4096
-    A = (torch.randn((K, M), dtype=torch.float32, device="cuda") / 45.25).T  # N(0,1)/sqrt(2048), column-major
-    B = (torch.randn((N, K), dtype=torch.float32, device="cuda") / 45.25).T
-    W = torch.randn((N, M), dtype=torch.float32, device="cuda").T
+    # N(0,1)/sqrt(2048) and N(0,1), column-major, drawn in HBM (device Box-Muller)
+    A = jb.fill_normal_(torch.empty((K, M), dtype=torch.float32, device="cuda"), 42, 2048 ** -0.5).T
+    B = jb.fill_normal_(torch.empty((N, K), dtype=torch.float32, device="cuda"), 43, 2048 ** -0.5).T
+    W = jb.fill_normal_(torch.empty((N, M), dtype=torch.float32, device="cuda"), 45).T
     Cm = jb.device_matrix(M, N, torch.float32)
     dA = jb.device_matrix(M, K, torch.float32)
     dB = jb.device_matrix(K, N, torch.float32)
