@@ -1199,13 +1199,14 @@ int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& f
     return fail(JB_ERR_UNSUPPORTED, "grouped plan: %zu work items is too many for one launch", nall);
   // pass 4: work queues.  One launch per 128-row/column tile shape and
   // transpose combination, then one for the other items (NLAUNCH); each
-  // launch has one queue per XCD.  A problem with at least 2 NXCD items is cut into NXCD runs of
-  // consecutive items, one per queue: items are ordered split by split, and
-  // within a split tile by tile in groups of `gr` tile-rows (column by column
-  // inside a group), so a run is a compact block of tiles sharing A row
-  // panels and B column panels.  A smaller problem goes whole to the queue of
-  // its launch with the least work so far (problems visited in decreasing
-  // order of their work).  Each queue runs its longest items first.
+  // launch has one queue per XCD.  A problem with at least 2 NXCD items is
+  // cut into NXCD runs of consecutive items, one per queue: items are ordered
+  // split by split, and within a split tile by tile in groups of `gr`
+  // tile-rows (column by column inside a group), so a run is a compact block
+  // of tiles sharing A row panels and B column panels.  A smaller problem goes
+  // whole to the queue of its launch with the least work so far (problems
+  // visited in decreasing order of their work).  Each queue runs its longest
+  // items first.
   std::vector<std::pair<double, GItem>> q[NLAUNCH][NXCD];
   double load[NLAUNCH][NXCD] = {};
   std::vector<int> order(probs.size());
