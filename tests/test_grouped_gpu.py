@@ -187,6 +187,46 @@ def test_grouped_large_tile_bitexact(gpu, ta, tb, tile, splitk, tmp_path):
         jb.tune_reset()
 
 
+def test_grouped_concurrent_launches_mixed_ops(gpu):
+    """One group whose problems need every persistent kernel at once: FP64
+    problems of all four transpose combinations on the 128x128, 128x64 and
+    64x128 tiles (twelve kernels) plus small-path and 32x32 problems, few
+    enough that the plan also splits k to fill the device; the launches run
+    concurrently on forked streams and join back.  Bit-exact per problem
+    (eager launch and graph replays)."""
+    torch = gpu
+    jb.tune_reset()
+    base = [(1024, 1024, 700), (1024, 64, 900), (64, 1024, 600), (40, 50, 60), (80, 90, 300)]
+    probs = []
+    for i in range(4 * len(base)):
+        (m, n, k), (ta, tb) = base[i % len(base)], [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")][i // len(base)]
+        A = dev_uniform(torch, m if ta == "N" else k, k if ta == "N" else m, 500 + 3 * i, torch.float64)
+        B = dev_uniform(torch, k if tb == "N" else n, n if tb == "N" else k, 501 + 3 * i, torch.float64)
+        C = dev_uniform(torch, m, n, 502 + 3 * i, torch.float64)
+        probs.append((ta, tb, m, n, k, A, B, C))
+    descs = [GemmDesc(ta, tb, 1.5, A, B, 0.5, C) for (ta, tb, m, n, k, A, B, C) in probs]
+    choices = jb.grouped_choices("f64", descs)
+    kinds = {c[0] for c in choices}
+    assert {"128x128", "128x64", "64x128", "small"} <= kinds, kinds
+    assert any(c[1] > 1 for c in choices), choices  # device-fill split-K took part
+    C0 = [to_host(p[7]) for p in probs]
+    refs = [O.gemm_recurrence_chunked(ta, tb, 1.5, to_host(A), to_host(B), 0.5, C0[i], choices[i][2])
+            for i, (ta, tb, m, n, k, A, B, C) in enumerate(probs)]
+    jb.gemm_grouped_("f64", descs)
+    torch.cuda.synchronize()
+    for i, p in enumerate(probs):
+        assert np.array_equal(to_host(p[7]), refs[i]), (i, p[:5], choices[i])
+    plan = jb.GroupedPlan("f64", descs)
+    for rep in range(2):
+        for i, p in enumerate(probs):
+            p[7].copy_(to_dev(torch, C0[i]))
+        plan.launch()
+        torch.cuda.synchronize()
+        for i, p in enumerate(probs):
+            assert np.array_equal(to_host(p[7]), refs[i]), (rep, i, p[:5], choices[i])
+    plan.destroy()
+
+
 SMALL_SHAPES = [(1, 1, 1), (64, 64, 64), (33, 17, 5), (7, 64, 63), (64, 3, 1), (50, 40, 64)]
 TILE_SHAPES = [(200, 72, 96), (16, 16, 2048)]
 
