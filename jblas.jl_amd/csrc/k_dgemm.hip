@@ -132,7 +132,7 @@ struct TileLoader {
 // ---------------------------------------------------------------------------
 // (1) MFMA GEMM: 128x128 block tile, 4 waves of 64x64 (4x4 MFMA blocks of
 //     16x16), or 64x64 with waves of 32x32 when the 128-tile grid would
-//     leave the device's last round mostly idle, or (FP64) 64x32 when even
+//     leave the device's last round mostly idle, or (FP64) 64x32 / 32x32 when even
 //     the 64-tiles are fewer than four per CU; BK=16 per LDS stage, two
 //     LDS stages; global loads are issued one full stage (128-tile) or four
 //     stages (64-tile) before their LDS store (the load latency, not the MFMA
@@ -494,21 +494,30 @@ static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s, int acc_fro
   const int64_t tiles128 = ((g.m + 127) / 128) * ((g.n + 127) / 128);
   const int64_t res = resident_128();
   const int64_t slots = ((tiles128 + res - 1) / res) * res;
-  const char* force = getenv("JB_GEMM_TILE");  // testing: pin one variant ("128", "64", "64x32")
+  // testing: pin one variant ("128", "64", "64x32", "32")
+  const char* force = getenv("JB_GEMM_TILE");
   if (force && *force) {
     if (!strcmp(force, "128")) return launch_gemm_mfma_bt<T, 128, 128>(g, s, acc_from_c);
     if (!strcmp(force, "64")) return launch_gemm_mfma_bt<T, 64, 64>(g, s, acc_from_c);
-    if constexpr (sizeof(T) == 8)  // FP32 64 x 32 would leave threads without a whole vector
+    if constexpr (sizeof(T) == 8) {  // FP32 64 x 32 and 32 x 32 would leave threads without a whole vector
       if (!strcmp(force, "64x32")) return launch_gemm_mfma_bt<T, 64, 32>(g, s, acc_from_c);
+      if (!strcmp(force, "32")) return launch_gemm_mfma_bt<T, 32, 32>(g, s, acc_from_c);
+    }
   }
   if ((double)tiles128 >= 0.9 * (double)slots) return launch_gemm_mfma_bt<T, 128, 128>(g, s, acc_from_c);
   // FP64 with fewer than four 64-tiles per CU: 64 x 32 tiles (two per CU hide
   // each other's barrier and LDS waits; measured +40 % at 512^3, +10 % at
-  // 1536^3, +3 % at 1024^3, even at 2048^3 where the 64-tile grid has 4 per CU)
+  // 1536^3, even at 2048^3 where the 64-tile grid has 4 per CU), and 32 x 32
+  // tiles where even the 64 x 32 grid has fewer than four per CU (waves of
+  // 16 x 16, four workgroups per CU: +31 % at 512^3, +25 % at 768^3, +4 % at
+  // 1024^3 over 64 x 32)
   if constexpr (sizeof(T) == 8) {
+    const int64_t cus4 = 4 * (int64_t)std::max(runtime_compute_units(), 1);
     const int64_t tiles64 = ((g.m + 63) / 64) * ((g.n + 63) / 64);
-    if (tiles64 < 4 * (int64_t)std::max(runtime_compute_units(), 1))
-      return launch_gemm_mfma_bt<T, 64, 32>(g, s, acc_from_c);
+    const int64_t tiles64x32 = ((g.m + 63) / 64) * ((g.n + 31) / 32);
+    if (tiles64 < cus4)
+      return tiles64x32 < cus4 ? launch_gemm_mfma_bt<T, 32, 32>(g, s, acc_from_c)
+                               : launch_gemm_mfma_bt<T, 64, 32>(g, s, acc_from_c);
   }
   return launch_gemm_mfma_bt<T, 64, 64>(g, s, acc_from_c);
 }

@@ -181,7 +181,7 @@ def test_fastmul_sign_of_zero_and_neighbours(gpu):
     assert np.all(buf[M * P:].cpu().numpy() == 7.0)
 
 
-@pytest.mark.parametrize("tile", ["128", "64", "64x32"])
+@pytest.mark.parametrize("tile", ["128", "64", "64x32", "32"])
 def test_native_tile_variants_bitexact(gpu, tile, monkeypatch):
     """Each MFMA tile variant (JB_GEMM_TILE pins it; the size rule picks it in
     normal use) is bit-identical to the recurrence, with fringes in m, n and k,

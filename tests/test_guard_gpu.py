@@ -73,7 +73,7 @@ def test_gemm_guard_bands(gpu, m, n, k, method, dtype):
                 g.check(f"{method} {dtype.__name__} {ta}{tb} {m}x{n}x{k} beta={beta}")
 
 
-@pytest.mark.parametrize("tile", ["128", "64", "64x32"])
+@pytest.mark.parametrize("tile", ["128", "64", "64x32", "32"])
 def test_mfma_tile_variants_guard_bands(gpu, tile, monkeypatch):
     torch = gpu
     monkeypatch.setenv("JB_GEMM_TILE", tile)
