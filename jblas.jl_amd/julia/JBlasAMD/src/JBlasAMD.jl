@@ -35,7 +35,8 @@ module JBlasAMD
 
 using LinearAlgebra
 
-export jmul!, fastmul!, gemm!, gemm, kernel!, initkernel!, Kernel, GroupedPlan, launch!, destroy!, nonfinite_check
+export jmul!, fastmul!, gemm!, gemm, kernel!, initkernel!, Kernel, GroupedPlan, launch!, destroy!, nonfinite_check,
+       gemm_grouped!, autotune!, tune_save, tune_load, tune_reset, tune_lookup, grouped_choices
 
 const libjblas = get(ENV, "JB_LIBRARY", normpath(joinpath(@__DIR__, "..", "..", "..", "lib", "libjblas_hip.so")))
 
@@ -338,6 +339,41 @@ function destroy!(p::GroupedPlan)
         p.handle = C_NULL
     end
     nothing
+end
+
+"All problems in one set of persistent launches, without a graph (synchronous on `stream`)."
+gemm_grouped!(method::Cint, descs::Vector{GemmDesc}; stream::Ptr{Cvoid} = C_NULL) =
+    check(:jb_gemm_grouped_dev, ccall((:jb_gemm_grouped_dev, libjblas), Cint,
+                                      (Cint, Int64, Ptr{GemmDesc}, Ptr{Cvoid}), method, length(descs), descs, stream))
+
+# ---- the autotuner (replaces pick_kernel_size / blocking_structure's static choice) ----
+const TILE_NAMES = Dict(0 => "64x64", 1 => "32x32", 2 => "small", 3 => "128x128", 4 => "128x64", 5 => "64x128")
+
+"Tune tile shape and split-K per size bucket of `descs`, timing the whole group (the C matrices are not touched)."
+autotune!(method::Cint, descs::Vector{GemmDesc}; reps::Integer = 3, stream::Ptr{Cvoid} = C_NULL) =
+    check(:jb_autotune_grouped, ccall((:jb_autotune_grouped, libjblas), Cint,
+                                      (Cint, Int64, Ptr{GemmDesc}, Cint, Ptr{Cvoid}),
+                                      method, length(descs), descs, Cint(reps), stream))
+tune_save(path::AbstractString) = check(:jb_autotune_save, ccall((:jb_autotune_save, libjblas), Cint, (Cstring,), path))
+tune_load(path::AbstractString) = check(:jb_autotune_load, ccall((:jb_autotune_load, libjblas), Cint, (Cstring,), path))
+tune_reset() = check(:jb_autotune_reset, ccall((:jb_autotune_reset, libjblas), Cint, ()))
+
+"(tile name, split-K) the table or the static rule gives one problem shape."
+function tune_lookup(method::Cint, m::Integer, n::Integer, k::Integer)
+    t, s = Ref{Cint}(0), Ref{Cint}(0)
+    check(:jb_autotune_lookup, ccall((:jb_autotune_lookup, libjblas), Cint,
+                                     (Cint, Int64, Int64, Int64, Ref{Cint}, Ref{Cint}), method, m, n, k, t, s))
+    (TILE_NAMES[Int(t[])], Int(s[]))
+end
+
+"Per problem, the (tile name, split-K, k per split) a plan of `descs` would use now (`nothing` for m or n == 0)."
+function grouped_choices(method::Cint, descs::Vector{GemmDesc})
+    n = length(descs)
+    t, s, kc = zeros(Cint, n), zeros(Cint, n), zeros(Int64, n)
+    check(:jb_grouped_choices, ccall((:jb_grouped_choices, libjblas), Cint,
+                                     (Cint, Int64, Ptr{GemmDesc}, Ptr{Cint}, Ptr{Cint}, Ptr{Int64}),
+                                     method, n, descs, t, s, kc))
+    [t[i] < 0 ? nothing : (TILE_NAMES[Int(t[i])], Int(s[i]), Int(kc[i])) for i in 1:n]
 end
 
 end # module
