@@ -157,10 +157,53 @@ __global__ __launch_bounds__(256) void absmax_kcontig_kernel(const double* __res
   }
 }
 
+// k contiguous with unit stride, 16-byte aligned rows: each lane loads two
+// doubles per instruction, four rows' loads in flight per wave (the general
+// form above issues 8-byte loads through a runtime stride).
+__global__ __launch_bounds__(256) void absmax_kunit_kernel(const double* __restrict__ P, int64_t sr, int64_t R,
+                                                           int64_t K, unsigned long long* __restrict__ maxbits) {
+  typedef double V2 __attribute__((ext_vector_type(2)));
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int64_t rw = (int64_t)blockIdx.x * (4 * AM_WROWS) + (int64_t)wave * AM_WROWS;
+  const int64_t k0 = (int64_t)blockIdx.y * AM_KK;
+  const bool full = k0 + AM_KK <= K;
+  for (int i0 = 0; i0 < AM_WROWS; i0 += 4) {
+    unsigned long long m[4] = {0ull, 0ull, 0ull, 0ull};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t r = rw + i0 + i;
+      if (r >= R) break;  // wave-uniform
+      const double* __restrict__ p = P + r * sr + k0;
+      if (full) {
+        V2 v[AM_KK / 128];
+#pragma unroll
+        for (int j = 0; j < AM_KK / 128; ++j) v[j] = *reinterpret_cast<const V2*>(p + 2 * lane + 128 * j);
+#pragma unroll
+        for (int j = 0; j < AM_KK / 128; ++j) m[i] = max(m[i], max(abs_bits(v[j][0]), abs_bits(v[j][1])));
+      } else {
+        for (int64_t k = lane; k0 + k < K && k < AM_KK; k += 64) m[i] = max(m[i], abs_bits(p[k]));
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t r = rw + i0 + i;
+      if (r >= R) break;
+      unsigned long long x = m[i];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) x = max(x, (unsigned long long)__shfl_xor(x, off, 64));
+      if (lane == 0) atomicMax(maxbits + r, x);
+    }
+  }
+}
+
 void launch_absmax(const double* P, int64_t sr, int64_t sk, int64_t R, int64_t K, unsigned long long* mb,
                    hipStream_t st) {
   if (R <= 0 || K <= 0) return;
-  if (sr == 1)
+  if (sr != 1 && sk == 1 && (reinterpret_cast<uintptr_t>(P) & 15) == 0 && sr % 2 == 0)
+    hipLaunchKernelGGL(absmax_kunit_kernel,
+                       dim3((unsigned)((R + 4 * AM_WROWS - 1) / (4 * AM_WROWS)), (unsigned)((K + AM_KK - 1) / AM_KK)),
+                       dim3(256), 0, st, P, sr, R, K, mb);
+  else if (sr == 1)
     hipLaunchKernelGGL(absmax_rows_kernel, dim3((unsigned)((R + 255) / 256), (unsigned)((K + AM_RKC - 1) / AM_RKC)),
                        dim3(256), 0, st, P, sk, R, K, mb);
   else
