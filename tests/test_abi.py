@@ -169,9 +169,8 @@ L.jb_dgemm(b"N", b"N", -3, 4, 4, ctypes.c_double(1.0), None, 1, None, 1, ctypes.
 
 @pytest.mark.parametrize("mode", ["file", "off"])
 def test_jb_log(tmp_path, mode):
-    """JB_LOG: quiet unless set; set to a path, every error an entry point
-    returns is appended there as one line (SURVEY: logging stays quiet unless
-    JB_LOG is set)."""
+    """JB_LOG: the library prints nothing unless it is set; set to a path,
+    every error an entry point returns is appended there as one line."""
     import subprocess
     import sys
     log = tmp_path / "jb.log"
