@@ -310,10 +310,10 @@ int jb_autotune_load(const char* path);
 /* Query the choice for one problem shape on its own: tile id (0 = 64x64 MFMA
  * tile, 1 = 32x32 MFMA tile, 2 = the whole problem in one work item on the
  * vector ALUs for m, n, k <= 64 -- fastmul!'s form, src/kernels.jl:202-208 --,
- * and, FP64 only, each run by a persistent kernel of its own: 3 = 128x128,
- * 4 = 128x64, 5 = 64x128 MFMA tiles) and split-K, from the table or the
- * static rule (FP64: the tile with the least padded area per unit of its
- * measured rate).  A plan may raise the split of
+ * and, FP64 and 3xBF16 only, each run by a persistent kernel of its own:
+ * 3 = 128x128, 4 = 128x64, 5 = 64x128 MFMA tiles) and split-K, from the table
+ * or the static rule (FP64: the tile with the least padded area per unit of its
+ * measured rate; 3xBF16: 64x128; native FP32: 64x64 or 32x32).  A plan may raise the split of
  * statically chosen problems while the group has fewer work items than the
  * device holds resident workgroups (jb_grouped_choices reports the result). */
 int jb_autotune_lookup(int method, int64_t m, int64_t n, int64_t k, int* tile, int* splitk);

@@ -10,14 +10,16 @@
 //     are pulled from a device counter by a grid of CU-filling workgroups,
 //     so uneven problems load-balance;
 //   * tile shapes 64x64 and 32x32 (4 waves each) are device functions of
-//     the same kernel, chosen per work item;
+//     one kernel, chosen per work item; the 128-row/column tiles (FP64 and
+//     3xBF16) run in kernels of their own, one per tile shape and transpose
+//     combination, launched concurrently with it;
 //   * split-K partials go to a workspace slot per (tile, split); the last
 //     split to finish (per-tile arrival counter) sums them in split order,
 //     so results are deterministic;
 //   * methods: FP64 on FP64 MFMA, FP32 on FP32 MFMA, FP32 as 3xBF16 (split on
 //     the fly into hi/mid/lo BF16 planes in LDS, 6 products on BF16 MFMA).
 // A plan uploads descriptors once and captures [counter reset kernel,
-// persistent kernel] in a hipGraph; launching the plan replays the graph.
+// persistent kernels] in a hipGraph; launching the plan replays the graph.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -41,9 +43,9 @@ constexpr int GNT = 256;  // 4 waves
 // (Operand and result pointers come from problem descriptors in memory: every
 // access goes through as_global, jb_internal.h.)
 // Work-item configurations: 0 = 64x64 MFMA tile, 1 = 32x32 MFMA tile, 2 = the
-// whole problem on the vector ALUs (m, n, k <= SMALL_MAX, one split); FP64
-// only, run by a second persistent kernel whose register budget is the
-// 128x128 tile's (so the others keep four workgroups per CU): 3 = 128x128,
+// whole problem on the vector ALUs (m, n, k <= SMALL_MAX, one split); FP64 and
+// 3xBF16 only, run by persistent kernels of their own whose register budget is
+// the large tile's (so the others keep four workgroups per CU): 3 = 128x128,
 // 4 = 128x64, 5 = 64x128 MFMA tiles.
 constexpr int NCFG = 6;
 constexpr int CFG_BM[NCFG] = {64, 32, 64, 128, 128, 64};  // tile rows
@@ -270,7 +272,7 @@ __device__ __forceinline__ void put_result(const GProb& P, int tm, int tn, int s
 
 // After every thread stored its partials: the last split of the tile to
 // arrive sums all partials in split order and applies the epilogue.
-template <typename T, int BM, int BN>
+template <typename T, int BM, int BN, int NTH = GNT>
 __device__ void splitk_combine(const GProb& P, int tm, int tn, int* counters, int* sflag) {
   if (P.splitk == 1) return;
   const int tile = tm * P.tiles_n + tn;
@@ -284,7 +286,7 @@ __device__ void splitk_combine(const GProb& P, int tm, int tn, int* counters, in
   gptr<T> C = as_global(static_cast<T*>(P.C));
   const T alpha = (T)P.alpha, beta = (T)P.beta;
   const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
-  for (int e = threadIdx.x; e < BM * BN; e += GNT) {
+  for (int e = threadIdx.x; e < BM * BN; e += NTH) {
     const int i = e % BM, j = e / BM;
     const int64_t m = m0 + i, n = n0 + j;
     if (m < P.m && n < P.n) {
@@ -512,21 +514,23 @@ __device__ void large_tile_f64(const GProb& P, int tm, int tn, int split, char* 
 __device__ __forceinline__ int bf_chunk(int r, int c) { return c ^ ((r >> 2) & 3); }
 
 // FP32 operand staging of the 3xBF16 tile in runs of four consecutive k of one
-// row (R x BK tile, NR runs per thread), split into hi/mid/lo BF16 planes
-// ([plane][row][BK], chunks swizzled) with one 8-byte LDS store per plane and run.
-//   RC (rows contiguous): run q of thread t is row t % R, k-run t / R + (GNT / R) q
-//   otherwise:            row t / (BK / 4) + (4 GNT / BK) q, k-run t % (BK / 4)
+// row (R x BK tile, NR runs per thread, NTH threads), split into hi/mid/lo BF16
+// planes ([plane][row][BK], chunks swizzled) with one 8-byte LDS store per plane
+// and run.
+//   RC (rows contiguous): run q of thread t is row t % R, k-run t / R + (NTH / R) q
+//   otherwise:            row t / (BK / 4) + (4 NTH / BK) q, k-run t % (BK / 4)
 // Lanes with consecutive t take consecutive addresses of the contiguous index.
-template <int R, int BK, bool RC>
+template <int R, int BK, bool RC, int NTH = GNT>
 struct GRun4 {
-  static constexpr int NR = R * BK / (4 * GNT);
-  static_assert(NR >= 1 && R * BK == 4 * GNT * NR, "tile must split into whole runs per thread");
-  static_assert(RC ? (GNT % R == 0 && (GNT / R) * NR == BK / 4) : ((4 * GNT / BK) * NR == R), "run map");
+  static constexpr int NR = R * BK / (4 * NTH);
+  static_assert(NR >= 1 && R * BK == 4 * NTH * NR, "tile must split into whole runs per thread");
+  static_assert(RC ? (NTH % R == 0 && (NTH / R) * NR == BK / 4) : ((4 * NTH / BK) * NR == R), "run map");
+  typedef float V4 __attribute__((ext_vector_type(4)));
   float v[NR][4];
   int tid;
   __device__ explicit GRun4(int t) : tid(t) {}
-  __device__ __forceinline__ int row(int q) const { return RC ? tid % R : tid / (BK / 4) + (4 * GNT / BK) * q; }
-  __device__ __forceinline__ int krun(int q) const { return RC ? tid / R + (GNT / R) * q : tid % (BK / 4); }
+  __device__ __forceinline__ int row(int q) const { return RC ? tid % R : tid / (BK / 4) + (4 * NTH / BK) * q; }
+  __device__ __forceinline__ int krun(int q) const { return RC ? tid / R + (NTH / R) * q : tid % (BK / 4); }
   __device__ __forceinline__ void load(gptr<const float> P, int64_t ld, int64_t r0, int64_t k0, int64_t Rtot,
                                        int64_t Kend) {
 #pragma unroll
@@ -535,6 +539,25 @@ struct GRun4 {
 #pragma unroll
       for (int e = 0; e < 4; ++e)
         v[q][e] = (gr < Rtot && gk + e < Kend) ? (RC ? P[gr + (gk + e) * ld] : P[gk + e + gr * ld]) : 0.0f;
+    }
+  }
+  // The whole R x BK tile inside the operand: no guards; a k-contiguous run is
+  // one 16-byte load when `vec` (16-byte aligned base, ld a multiple of 4).
+  __device__ __forceinline__ void load_full(gptr<const float> P, int64_t ld, int64_t r0, int64_t k0, bool vec) {
+    if (!RC && vec) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        const V4 x = *(gptr<const V4>)(P + (k0 + 4 * krun(q)) + (r0 + row(q)) * ld);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[q][e] = x[e];
+      }
+      return;
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const int64_t gr = r0 + row(q), gk = k0 + 4 * krun(q);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[q][e] = RC ? P[gr + (gk + e) * ld] : P[gk + e + gr * ld];
     }
   }
   // x = hi + mid + lo + rest: each plane is x rounded to BF16 (nearest even,
@@ -651,6 +674,121 @@ __device__ void bf16x3_tile(const GProb& P, int tm, int tn, int split, char* sme
         put_result<float, BM, BN>(P, tm, tn, split, wm0 + 16 * i + l16, wn0 + 16 * j + 4 * g + r,
                                   acc[0][i][j][r] + acc[1][i][j][r]);
   splitk_combine<float, BM, BN>(P, tm, tn, counters, sflag);
+}
+
+// 3xBF16 tile of the 128-row/column shapes (128x128, 128x64, 64x128): WM x WN
+// waves of (BM/WM) x (BN/WN), each a grid of 32x32 blocks of
+// v_mfma_f32_32x32x16_bf16 (one 16-byte fragment per lane feeds twice the
+// products of a 16x16x32 fragment, so the LDS reads per product halve).  The
+// staging and the LDS image are those of bf16x3_tile ([plane][row][32 k],
+// chunks swizzled by bf_chunk); a 32x32x16 fragment of k-step h is chunk
+// 2h + lane/32 of row lane%32.  Same 6 products and two accumulators.  The
+// in-register split costs VALU work per staged element, so the larger tiles
+// (fewer staged elements per product) run more MFMAs per VALU instruction.
+template <int BM, int BN, int WM, int WN, bool TA, bool TB>
+__device__ void x3_large_tile(const GProb& P, int tm, int tn, int split, char* smem, int* counters, int* sflag) {
+  constexpr int NTH = 64 * WM * WN;
+  constexpr int BK = 32;
+  constexpr int RS = BK;
+  constexpr int IB = BM / WM / 32, JB = BN / WN / 32;  // 32x32 blocks per wave along m, n
+  static_assert(IB >= 1 && JB >= 1 && IB * WM * 32 == BM && JB * WN * 32 == BN, "wave grid");
+  constexpr int BUF = 3 * (BM + BN) * RS;
+  typedef float f16v __attribute__((ext_vector_type(16)));
+  uint16_t* const S0 = reinterpret_cast<uint16_t*>(smem);
+  const int tid = work_item_tid();
+  const int lane = tid % 64, wave = tid / 64;
+  const int wm0 = (wave % WM) * (BM / WM), wn0 = (wave / WM) * (BN / WN);
+  const int l32 = lane % 32, hl = lane / 32;
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+  const int64_t kb = (int64_t)split * P.kchunk;
+  const int64_t ke = min(P.k, kb + P.kchunk);
+  gptr<const float> A = as_global(static_cast<const float*>(P.A));
+  gptr<const float> B = as_global(static_cast<const float*>(P.B));
+  GRun4<BM, BK, !TA, NTH> la(tid);
+  GRun4<BN, BK, TB, NTH> lb(tid);
+  // the full-tile loads: rows inside the operand; k-contiguous operands in
+  // 16-byte vectors where the base is 16-byte aligned and ld a multiple of 4
+  const bool fa = m0 + BM <= P.m, fb = n0 + BN <= P.n;
+  const bool va = ((reinterpret_cast<uintptr_t>(P.A) & 15) == 0) && P.lda % 4 == 0;
+  const bool vb = ((reinterpret_cast<uintptr_t>(P.B) & 15) == 0) && P.ldb % 4 == 0;
+  auto load_stage = [&](int64_t k0) {
+    if (fa && fb && k0 + BK <= ke) {
+      la.load_full(A, P.lda, m0, k0, va);
+      lb.load_full(B, P.ldb, n0, k0, vb);
+    } else {
+      la.load(A, P.lda, m0, k0, P.m, ke);
+      lb.load(B, P.ldb, n0, k0, P.n, ke);
+    }
+  };
+  f16v acc[2][IB][JB];
+#pragma unroll
+  for (int l = 0; l < 2; ++l)
+#pragma unroll
+    for (int i = 0; i < IB; ++i)
+#pragma unroll
+      for (int j = 0; j < JB; ++j)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[l][i][j][e] = 0.0f;
+  const int nkt = (int)((ke - kb + BK - 1) / BK);
+  // two LDS stages, one barrier per stage, as in bf16x3_tile
+  if (nkt > 0) {
+    __syncthreads();  // LDS may still be read by the previous work item
+    load_stage(kb);
+    la.store3(S0);
+    lb.store3(S0 + 3 * BM * RS);
+    if (nkt > 1) load_stage(kb + BK);
+    __syncthreads();
+  }
+  for (int kt = 0; kt < nkt; ++kt) {
+    const uint16_t* As = S0 + (kt & 1) * BUF;
+    const uint16_t* Bs = As + 3 * BM * RS;
+#pragma unroll
+    for (int h = 0; h < BK / 16; ++h) {
+      bf8 a[3][IB], b[3][JB];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+#pragma unroll
+        for (int i = 0; i < IB; ++i) {
+          const int r = wm0 + 32 * i + l32;
+          a[p][i] = __builtin_bit_cast(bf8, *reinterpret_cast<const u4*>(As + (p * BM + r) * RS + 8 * bf_chunk(r, 2 * h + hl)));
+        }
+#pragma unroll
+        for (int j = 0; j < JB; ++j) {
+          const int r = wn0 + 32 * j + l32;
+          b[p][j] = __builtin_bit_cast(bf8, *reinterpret_cast<const u4*>(Bs + (p * BN + r) * RS + 8 * bf_chunk(r, 2 * h + hl)));
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < IB; ++i)
+#pragma unroll
+        for (int j = 0; j < JB; ++j) {
+          // X = op(B) plane (p -> n), Y = op(A) plane (q -> m)
+          acc[0][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[0][j], a[0][i], acc[0][i][j], 0, 0, 0);
+          acc[1][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[1][j], a[0][i], acc[1][i][j], 0, 0, 0);
+          acc[1][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[0][j], a[1][i], acc[1][i][j], 0, 0, 0);
+          acc[1][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[2][j], a[0][i], acc[1][i][j], 0, 0, 0);
+          acc[1][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[1][j], a[1][i], acc[1][i][j], 0, 0, 0);
+          acc[1][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[0][j], a[2][i], acc[1][i][j], 0, 0, 0);
+        }
+    }
+    if (kt + 1 < nkt) {
+      uint16_t* nxt = S0 + ((kt + 1) & 1) * BUF;  // last read in stage kt-1, before the previous barrier
+      la.store3(nxt);
+      lb.store3(nxt + 3 * BM * RS);
+      if (kt + 2 < nkt) load_stage(kb + (int64_t)(kt + 2) * BK);
+    }
+    __syncthreads();
+  }
+  // item e of lane l in block (i, j): row wm0 + 32 i + l%32, column wn0 + 32 j + 8 (e/4) + 4 (l/32) + e%4
+#pragma unroll
+  for (int i = 0; i < IB; ++i)
+#pragma unroll
+    for (int j = 0; j < JB; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e)
+        put_result<float, BM, BN>(P, tm, tn, split, wm0 + 32 * i + l32, wn0 + 32 * j + 8 * (e / 4) + 4 * hl + e % 4,
+                                  acc[0][i][j][e] + acc[1][i][j][e]);
+  splitk_combine<float, BM, BN, NTH>(P, tm, tn, counters, sflag);
 }
 
 // Small problem (m, n, k <= 64) in one work item, the form of fastmul!
@@ -876,6 +1014,33 @@ __global__ __launch_bounds__(GNT, 2) void grouped_large_kernel(const GProb* __re
   }
 }
 
+// The 3xBF16 counterpart: items of one 128-row/column tile shape and transpose
+// combination.  LDS: two stages of three BF16 planes of (BM + BN) rows x 32 k.
+// 128x64 and 64x128: four waves of 64x32 / 32x64, 72 KB (two workgroups per
+// CU); 128x128: eight waves of 64x32, 96 KB (one workgroup of 8 waves per CU).
+template <int BM, int BN> struct X3Waves {
+  static constexpr int WM = 2, WN = (BM == 128 && BN == 128) ? 4 : 2;
+  static constexpr int NTH = 64 * WM * WN;
+  static constexpr int MINB = NTH > GNT ? 1 : 2;  // workgroups per CU for the register budget
+};
+template <int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__((X3Waves<BM, BN>::NTH), (X3Waves<BM, BN>::MINB))
+void grouped_large_x3_kernel(const GProb* __restrict__ probs, const GItem* __restrict__ items,
+                             const int* __restrict__ qdesc, int* __restrict__ qnext, int* __restrict__ counters) {
+  constexpr int SMEM = 2 * 3 * (BM + BN) * 32 * 2;
+  __shared__ __attribute__((aligned(16))) char smem[SMEM];
+  __shared__ int s_item, s_flag;
+  int q = blockIdx.x % NXCD, tries = 0;
+  for (;;) {
+    const int it = pop_item(qdesc, qnext, q, tries, &s_item);
+    if (it < 0) break;
+    const GItem w = items[it];
+    const GProb P = probs[w.prob];
+    const int tm = w.tile / P.tiles_n, tn = w.tile % P.tiles_n;
+    x3_large_tile<BM, BN, X3Waves<BM, BN>::WM, X3Waves<BM, BN>::WN, TA, TB>(P, tm, tn, w.split, smem, counters, &s_flag);
+  }
+}
+
 __global__ void zero_i32_kernel(int* p, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
@@ -917,12 +1082,17 @@ std::string bucket_key(int method, int64_t m, int64_t n, int64_t k) {
 // against 44.4 us for the tiles in FP64 and 34.7 against 37.6 us in 3xBF16);
 // otherwise, for FP64, the tile with the least padded area per unit of
 // measured throughput (TILE_RATE: each tile's FP64 rate on groups of
-// 1024 x 1024 problems with k = 256 and 1024, relative to the 128x128 tile),
-// and for FP32 the 64x64 tile from
-// max(m, n) >= 96, else 32x32.  Split-K is not chosen per problem: plan_build
-// splits the longest work items when the whole group has fewer work items than
-// the device holds workgroups (the blocking decision that depends on the
-// device, blocking_structure's role in src/memory_management.jl:78-140).
+// 1024 x 1024 problems with k = 256 and 1024, relative to the 128x128 tile);
+// for 3xBF16 the 64x128 tile (its split in registers makes every tile shape
+// cost about the same per staged element: on the 1024-problem log-uniform set
+// the 64x128 tile for every problem ran 7 % (NN) to 24 % (TN) faster than the
+// 64x64 / 32x32 rule and within 2 % of the best single shape per transpose
+// combination except NT, where 128x128 is 7.6 % faster -- a choice the tuner
+// can make); for native FP32 the 64x64 tile from max(m, n) >= 96, else 32x32.
+// Split-K is not chosen per problem: plan_build splits the longest work items
+// when the whole group has fewer work items than the device holds workgroups
+// (the blocking decision that depends on the device, blocking_structure's role
+// in src/memory_management.jl:78-140).
 constexpr double TILE_RATE[NCFG] = {0.90, 0.65, 0.0, 1.0, 1.0, 0.97};
 void default_choice(int kind, int64_t m, int64_t n, int64_t k, int& cfg, int& splitk) {
   splitk = 1;
@@ -930,8 +1100,12 @@ void default_choice(int kind, int64_t m, int64_t n, int64_t k, int& cfg, int& sp
     cfg = CFG_SMALL;
     return;
   }
-  if (kind != 0) {
+  if (kind == 1) {
     cfg = std::max(m, n) >= 96 ? 0 : 1;
+    return;
+  }
+  if (kind == 2) {
+    cfg = CFG_WIDE;
     return;
   }
   double best = 0;
@@ -948,13 +1122,13 @@ void default_choice(int kind, int64_t m, int64_t n, int64_t k, int& cfg, int& sp
 
 // A choice taken from a bucket (table, tuner trial, forced) may name the small
 // path for a problem that does not fit it: such a problem takes the 32x32 tile;
-// the 128-row/column tiles exist for FP64 only (others take 64x64).
+// the 128-row/column tiles exist for FP64 and 3xBF16 (native FP32 takes 64x64).
 void fit_choice(int kind, int64_t m, int64_t n, int64_t k, int& cfg, int& splitk) {
   if (cfg == CFG_SMALL) {
     if (m <= SMALL_MAX && n <= SMALL_MAX && k <= SMALL_MAX) splitk = 1;
     else cfg = 1;
   }
-  if (large_cfg(cfg) && kind != 0) cfg = 0;
+  if (large_cfg(cfg) && kind == 1) cfg = 0;
 }
 
 // Returns true when the choice is the static default (plan_build may then add split-K).
@@ -1050,28 +1224,46 @@ int validate_descs(int method, int64_t count, const jb_gemm_desc* d) {
 
 size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Resident workgroups per CU of the persistent kernel of `kind`, from the
-// device's register and LDS budget for that kernel (queried once per kind and
-// device, under the runtime lock; the persistent grid is this times the CU count).
-// kind 3: the 128x128 FP64 kernel.
+// Resident workgroups per CU of a persistent kernel, from the device's
+// register and LDS budget for that kernel (queried once per kernel and device,
+// under the runtime lock; the persistent grid is this times the CU count).
+// kind 0-2: the kernel of the 64x64 / 32x32 / small items of method kind 0-2;
+// 3: the 128-row/column FP64 kernels (the 128x128 one's budget); 4: the
+// 128x128 3xBF16 kernels; 5: the 128x64 and 64x128 3xBF16 kernels.
 hipError_t grouped_occupancy(int kind, int* occ) {
-  static int cached[4] = {0, 0, 0, 0};
+  static int cached[6] = {0, 0, 0, 0, 0, 0};
   static int cached_device = -1;
   if (cached_device != rt().device) {
-    cached[0] = cached[1] = cached[2] = cached[3] = 0;
+    for (int& c : cached) c = 0;
     cached_device = rt().device;
   }
   if (cached[kind] == 0) {
     int n = 0;
-    hipError_t e = kind == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<0>, GNT, 0)
-                   : kind == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<1>, GNT, 0)
-                   : kind == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<2>, GNT, 0)
-                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_large_kernel<128, 128, true, true>, GNT, 0);
+    hipError_t e;
+    switch (kind) {
+      case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<0>, GNT, 0); break;
+      case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<1>, GNT, 0); break;
+      case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<2>, GNT, 0); break;
+      case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_large_kernel<128, 128, true, true>, GNT, 0); break;
+      case 4:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_large_x3_kernel<128, 128, true, true>,
+                                                         X3Waves<128, 128>::NTH, 0);
+        break;
+      default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_large_x3_kernel<128, 64, true, true>, GNT, 0); break;
+    }
     if (e != hipSuccess) return e;
     cached[kind] = std::min(std::max(n, 1), 8);
   }
   *occ = cached[kind];
   return hipSuccess;
+}
+
+// The occupancy kind (grouped_occupancy) of launch L < NLAUNCH - 1 of a plan of
+// method kind `kind`, and of the 128-row/column kernels as a whole (L < 0: the
+// device fill of plan_choices).
+int large_occ_kind(int kind, int L) {
+  if (kind == 0) return 3;
+  return (L >= 0 && L < 4) ? 4 : 5;
 }
 
 // Builds the device-side problem/work lists of a plan (no launch).
@@ -1087,7 +1279,7 @@ int plan_choices(int method, int64_t count, const jb_gemm_desc* d, const Forced&
   const int kind = kind_of(method);
   int occ = 0, occ_large = 1;
   hipError_t e = grouped_occupancy(kind, &occ);
-  if (e == hipSuccess && kind == 0) e = grouped_occupancy(3, &occ_large);
+  if (e == hipSuccess && kind != 1) e = grouped_occupancy(large_occ_kind(kind, -1), &occ_large);
   if (e != hipSuccess) return hip_fail(e, "grouped plan: occupancy query");
   std::vector<char> dflt;
   *flops = 0;
@@ -1170,10 +1362,8 @@ int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& f
   P.kind = kind_of(method);
   const size_t elt = P.kind == 0 ? 8 : 4;
   hipError_t e;
-  int occ = 0, occ_large = 0;
+  int occ = 0;
   if ((e = grouped_occupancy(P.kind, &occ)) != hipSuccess) return hip_fail(e, "grouped plan: occupancy query");
-  if (P.kind == 0 && (e = grouped_occupancy(3, &occ_large)) != hipSuccess)
-    return hip_fail(e, "grouped plan: occupancy query");
   std::vector<GProb> probs;
   int rc = plan_choices(method, count, d, forced, probs, &P.flops);
   if (rc) return rc;
@@ -1279,7 +1469,9 @@ int plan_build(int method, int64_t count, const jb_gemm_desc* d, const Forced& f
       for (auto& w : q[L][x]) items.push_back(w.second);
       P.litems[L] += (int)q[L][x].size();
     }
-    const int occ_l = L < NLAUNCH - 1 ? std::max(occ_large, 1) : occ;
+    int occ_l = occ;
+    if (L < NLAUNCH - 1 && P.litems[L] > 0 && (e = grouped_occupancy(large_occ_kind(P.kind, L), &occ_l)) != hipSuccess)
+      return hip_fail(e, "grouped plan: occupancy query");
     P.lgrid[L] = std::max(1, std::min(P.litems[L], cus * occ_l));
   }
   if (!probs.empty() && (e = hipMemcpy(P.probs, probs.data(), probs.size() * sizeof(GProb), hipMemcpyHostToDevice)) != hipSuccess)
@@ -1330,6 +1522,13 @@ const GroupedKernelFn kLargeKernels[NLAUNCH - 1] = {
     grouped_large_kernel<128, 64, true, false>,   grouped_large_kernel<128, 64, true, true>,
     grouped_large_kernel<64, 128, false, false>,  grouped_large_kernel<64, 128, false, true>,
     grouped_large_kernel<64, 128, true, false>,   grouped_large_kernel<64, 128, true, true>};
+const GroupedKernelFn kLargeX3Kernels[NLAUNCH - 1] = {
+    grouped_large_x3_kernel<128, 128, false, false>, grouped_large_x3_kernel<128, 128, false, true>,
+    grouped_large_x3_kernel<128, 128, true, false>,  grouped_large_x3_kernel<128, 128, true, true>,
+    grouped_large_x3_kernel<128, 64, false, false>,  grouped_large_x3_kernel<128, 64, false, true>,
+    grouped_large_x3_kernel<128, 64, true, false>,   grouped_large_x3_kernel<128, 64, true, true>,
+    grouped_large_x3_kernel<64, 128, false, false>,  grouped_large_x3_kernel<64, 128, false, true>,
+    grouped_large_x3_kernel<64, 128, true, false>,   grouped_large_x3_kernel<64, 128, true, true>};
 
 // Enqueues [reset counters, 128-row/column persistent kernels, persistent kernel] on s.
 hipError_t plan_enqueue(const GroupedPlanImpl& P, hipStream_t s) {
@@ -1350,9 +1549,11 @@ hipError_t plan_enqueue(const GroupedPlanImpl& P, hipStream_t s) {
     }
     const int* qd = P.qdesc + L * 2 * NXCD;
     int* qn = P.next + L * NXCD;
-    const dim3 grid(P.lgrid[L]), block(GNT);
+    // the 128x128 3xBF16 kernels run workgroups of 8 waves
+    const dim3 grid(P.lgrid[L]), block(P.kind == 2 && L < 4 ? X3Waves<128, 128>::NTH : GNT);
     if (L < NLAUNCH - 1) {
-      hipLaunchKernelGGL(kLargeKernels[L], grid, block, 0, ls, P.probs, P.items, qd, qn, tiles);
+      hipLaunchKernelGGL(P.kind == 0 ? kLargeKernels[L] : kLargeX3Kernels[L], grid, block, 0, ls, P.probs, P.items, qd, qn,
+                         tiles);
     } else if (P.kind == 0) {
       hipLaunchKernelGGL(grouped_kernel<0>, grid, block, 0, ls, P.probs, P.items, qd, qn, tiles);
     } else if (P.kind == 1) {
@@ -1650,7 +1851,7 @@ int jb_autotune_grouped(int method, int64_t count, const jb_gemm_desc* descs, in
       for (int sk = 1; sk <= 16 && rc == JB_OK; sk *= 2) {
         if (sk > 1 && kmax / sk < 128) continue;
         if (cfg == CFG_SMALL && (sk > 1 || kmax > SMALL_MAX)) continue;  // one split; k must fit
-        if (large_cfg(cfg) && kind_of(method) != 0) continue;               // FP64 only
+        if (large_cfg(cfg) && kind_of(method) == 1) continue;               // FP64 and 3xBF16 only
         over[key] = TuneEntry{cfg, sk, 0.0};
         double t = 0;
         rc = autotune_trial(method, count, descs, over, reps, s, e0, e1, &t);

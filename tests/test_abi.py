@@ -195,9 +195,9 @@ def test_grouped_static_choice_rule():
     needed): problems with m, n, k <= 64 run whole in one work item (fastmul!'s
     form, src/kernels.jl:202-208); other FP64 problems take the tile with the
     least padded area divided by the tile's measured relative rate (128x128
-    and 128x64: 1, 64x128: 0.97, 64x64: 0.9, 32x32: 0.65); FP32 problems the
-    64x64 tile from max(m, n) >= 96, else 32x32; one split (a plan adds splits
-    to fill the device)."""
+    and 128x64: 1, 64x128: 0.97, 64x64: 0.9, 32x32: 0.65); 3xBF16 problems the
+    64x128 tile; native FP32 problems the 64x64 tile from max(m, n) >= 96, else
+    32x32; one split (a plan adds splits to fill the device)."""
     from jblas_amd import grouped as G
     G.tune_reset()
     for m, n, k in [(1, 1, 1), (64, 64, 64), (16, 40, 8), (64, 1, 64), (5, 7, 0)]:
@@ -217,6 +217,7 @@ def test_grouped_static_choice_rule():
                             (1000, 136, 300, "128x64"), (136, 1000, 300, "64x128")]:
         assert rule(m, n) == want, (m, n)
         assert G.tune_lookup("f64", m, n, k) == (want, 1), (m, n, k)
-    assert G.tune_lookup("f32", 1024, 1024, 1024) == ("64x64", 1)  # the 128-row/column tiles are FP64 only
-    assert G.tune_lookup("f32_bf16x3", 1024, 64, 512) == ("64x64", 1)
-    assert G.tune_lookup("f32_bf16x3", 65, 64, 64) == ("32x32", 1)
+    assert G.tune_lookup("f32", 1024, 1024, 1024) == ("64x64", 1)  # no 128-row/column tiles in native FP32
+    assert G.tune_lookup("f32", 65, 64, 64) == ("32x32", 1)
+    for m, n, k in [(1024, 1024, 1024), (1024, 64, 512), (65, 64, 64), (16, 16, 4096), (2048, 16, 16)]:
+        assert G.tune_lookup("f32_bf16x3", m, n, k) == ("64x128", 1), (m, n, k)

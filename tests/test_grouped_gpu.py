@@ -187,6 +187,64 @@ def test_grouped_large_tile_bitexact(gpu, ta, tb, tile, splitk, tmp_path):
         jb.tune_reset()
 
 
+def strided32(torch, rows, cols, seed, ld, offset):
+    """Float32 `strided`: ld a multiple of 4 at offset 0 gives 16-byte vectors,
+    an odd ld or offset element loads."""
+    buf = torch.empty(offset + ld * max(cols, 1), dtype=torch.float32, device="cuda")
+    jb.fill_uniform_(buf, seed)
+    return buf[offset:offset + ld * cols].view(cols, ld).T[:rows] if cols else buf[:0].view(0, rows).T
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("tile,splitk", [("128x128", 1), ("128x128", 3), ("128x64", 1), ("128x64", 2),
+                                         ("64x128", 1), ("64x128", 3)])
+def test_grouped_bf16x3_large_tiles(gpu, ta, tb, tile, splitk, tmp_path):
+    """The 3xBF16 tiles with 128 rows or columns (a persistent kernel per tile
+    shape and transpose combination; 128x128 with eight waves), forced through
+    a tuning table on fringe shapes (partial tiles in m, n and k; k == 0), with
+    the full-tile 16-byte loads (aligned bases, leading dimensions a multiple of
+    4) and the element-wise ones (odd leading dimension or offset), one split or
+    more: within 2*K*2^-24 componentwise of the FP64 (Dot2) oracle, and
+    beta*C for k == 0."""
+    torch = gpu
+    jb.tune_reset()
+    cfg = {v: k for k, v in jb.grouped.TILES.items()}[tile]
+    write_table(str(tmp_path / "large3.json"), "f32_bf16x3", LARGE_SHAPES, cfg, splitk)
+    try:
+        probs = []
+        for i, (m, n, k) in enumerate(LARGE_SHAPES):
+            ra, ca = (m, k) if ta == "N" else (k, m)
+            rb, cb = (k, n) if tb == "N" else (n, k)
+            odd = i % 2 == 1
+            A = strided32(torch, ra, ca, 400 + 3 * i, ra + 1 if odd else -(-ra // 4) * 4, 1 if odd else 0)
+            B = strided32(torch, rb, cb, 401 + 3 * i, rb + 3 if odd else -(-rb // 4) * 4 + 4, 0)
+            C = strided32(torch, m, n, 402 + 3 * i, m + 5, 3)
+            probs.append((m, n, k, A, B, C))
+        C0 = [to_host(p[5]) for p in probs]
+        for alpha, beta in [(1.5, 0.5), (-1.0, 0.0)]:
+            for i, p in enumerate(probs):
+                p[5].copy_(to_dev(torch, C0[i]))
+            descs = [GemmDesc(ta, tb, alpha, A, B, beta, C) for (m, n, k, A, B, C) in probs]
+            choices = jb.grouped_choices("f32_bf16x3", descs)
+            assert all(c[0] == tile for c in choices), choices
+            jb.gemm_grouped_("f32_bf16x3", descs)
+            torch.cuda.synchronize()
+            for i, (m, n, k, A, B, C) in enumerate(probs):
+                Ah, Bh, got = to_host(A), to_host(B), to_host(C).astype(float)
+                ref = O.sgemm_dot2(ta, tb, alpha, Ah, Bh, beta, C0[i])
+                Ao = Ah if ta == "N" else Ah.T
+                Bo = Bh if tb == "N" else Bh.T
+                bound = (abs(alpha) * (np.abs(Ao).astype(float) @ np.abs(Bo).astype(float))
+                         + abs(beta) * np.abs(C0[i]).astype(float))
+                if k == 0:  # C = beta*C exactly (+0 for beta == 0)
+                    assert np.array_equal(got, (np.float32(beta) * C0[i]).astype(float)), (ta, tb, m, n, alpha, beta)
+                else:
+                    err = np.max(np.abs(got - ref) / bound)
+                    assert err <= 2 * k * U32, (ta, tb, m, n, k, choices[i], alpha, beta, err)
+    finally:
+        jb.tune_reset()
+
+
 def test_grouped_concurrent_launches_mixed_ops(gpu):
     """One group whose problems need every persistent kernel at once: FP64
     problems of all four transpose combinations on the 128x128, 128x64 and
@@ -225,6 +283,62 @@ def test_grouped_concurrent_launches_mixed_ops(gpu):
         for i, p in enumerate(probs):
             assert np.array_equal(to_host(p[7]), refs[i]), (rep, i, p[:5], choices[i])
     plan.destroy()
+
+
+def test_grouped_bf16x3_concurrent_launches(gpu, tmp_path):
+    """3xBF16 group whose problems need every persistent kernel at once: a
+    tuning table puts one size bucket each on 128x128 (eight-wave workgroups,
+    two splits), 128x64, 64x128, 64x64, 32x32 and the small path, in all four
+    transpose combinations; the launches run concurrently on forked streams.
+    Every problem within 2*K*2^-24 componentwise of the FP64 (Dot2) oracle;
+    graph replays bit-identical to the eager launch."""
+    torch = gpu
+    jb.tune_reset()
+    base = [((1024, 1024, 700), 3, 2), ((1024, 64, 900), 4, 1), ((64, 1024, 300), 5, 1), ((200, 200, 100), 0, 1),
+            ((80, 90, 300), 1, 1), ((40, 50, 60), 2, 1)]
+    info = jb.device_info()
+    pre = f"{info['name']}|{info['arch']}|m{jb.grouped.method_id('f32_bf16x3')}|"
+    entries = ",\n".join(
+        f'{{"key": "{pre}mn{max(math.isqrt(m * n), 1).bit_length() - 1}|k{max(k, 1).bit_length() - 1}", '
+        f'"cfg": {cfg}, "splitk": {sk}, "ms": 0.0}}' for (m, n, k), cfg, sk in base)
+    path = tmp_path / "mixed3.json"
+    path.write_text('{"format": "jblas-tune-1", "entries": [' + entries + "]}")
+    jb.tune_load(str(path))
+    try:
+        probs = []
+        for i in range(4 * len(base)):
+            (m, n, k), (ta, tb) = base[i % len(base)][0], [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")][i // len(base)]
+            A = dev_uniform(torch, m if ta == "N" else k, k if ta == "N" else m, 700 + 3 * i, torch.float32)
+            B = dev_uniform(torch, k if tb == "N" else n, n if tb == "N" else k, 701 + 3 * i, torch.float32)
+            C = dev_uniform(torch, m, n, 702 + 3 * i, torch.float32)
+            probs.append((ta, tb, m, n, k, A, B, C))
+        descs = [GemmDesc(ta, tb, 1.5, A, B, 0.5, C) for (ta, tb, m, n, k, A, B, C) in probs]
+        choices = jb.grouped_choices("f32_bf16x3", descs)
+        want = {"128x128": 2, "128x64": 1, "64x128": 1, "64x64": 1, "32x32": 1, "small": 1}
+        assert [(c[0], c[1]) for c in choices[:len(base)]] == list(want.items()), choices
+        C0 = [to_host(p[7]) for p in probs]
+        jb.gemm_grouped_("f32_bf16x3", descs)
+        torch.cuda.synchronize()
+        eager = [to_host(p[7]) for p in probs]
+        for i, (ta, tb, m, n, k, A, B, C) in enumerate(probs):
+            Ah, Bh = to_host(A), to_host(B)
+            ref = O.sgemm_dot2(ta, tb, 1.5, Ah, Bh, 0.5, C0[i])
+            Ao = Ah if ta == "N" else Ah.T
+            Bo = Bh if tb == "N" else Bh.T
+            bound = 1.5 * (np.abs(Ao).astype(float) @ np.abs(Bo).astype(float)) + 0.5 * np.abs(C0[i]).astype(float)
+            err = np.max(np.abs(eager[i].astype(float) - ref) / bound)
+            assert err <= 2 * k * U32, (i, ta, tb, m, n, k, choices[i], err)
+        plan = jb.GroupedPlan("f32_bf16x3", descs)
+        for rep in range(2):
+            for i, p in enumerate(probs):
+                p[7].copy_(to_dev(torch, C0[i]))
+            plan.launch()
+            torch.cuda.synchronize()
+            for i, p in enumerate(probs):
+                assert np.array_equal(to_host(p[7]), eager[i]), (rep, i, p[:5], choices[i])
+        plan.destroy()
+    finally:
+        jb.tune_reset()
 
 
 SMALL_SHAPES = [(1, 1, 1), (64, 64, 64), (33, 17, 5), (7, 64, 63), (64, 3, 1), (50, 40, 64)]
