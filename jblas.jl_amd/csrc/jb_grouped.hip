@@ -541,23 +541,44 @@ struct GRun4 {
         v[q][e] = (gr < Rtot && gk + e < Kend) ? (RC ? P[gr + (gk + e) * ld] : P[gk + e + gr * ld]) : 0.0f;
     }
   }
-  // The whole R x BK tile inside the operand: no guards; a k-contiguous run is
-  // one 16-byte load when `vec` (16-byte aligned base, ld a multiple of 4).
-  __device__ __forceinline__ void load_full(gptr<const float> P, int64_t ld, int64_t r0, int64_t k0, bool vec) {
+  // Stages whose k range lies inside the operand: the lane part of each run's
+  // address is formed once per work item (lane_base), with the row clamped
+  // into the operand (a row past its end reads the last row and is zeroed);
+  // a stage then adds only the wave-uniform k0 (times ld for rows-contiguous
+  // operands), so a load costs one 64-bit add instead of a guarded 64-bit
+  // multiply-add chain per element.  A k-contiguous run is one 16-byte load
+  // when `vec` (16-byte aligned base, ld a multiple of 4).
+  gptr<const float> lp[NR];
+  bool rok[NR];
+  __device__ __forceinline__ void lane_base(gptr<const float> P, int64_t ld, int64_t r0, int64_t Rtot) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const int64_t r = r0 + row(q);
+      rok[q] = r < Rtot;
+      const int64_t rc = rok[q] ? r : Rtot - 1;
+      lp[q] = RC ? P + rc + (int64_t)(4 * krun(q)) * ld : P + rc * ld + 4 * krun(q);
+    }
+  }
+  // rows_in: every row of the tile lies inside the operand (no zeroing)
+  __device__ __forceinline__ void load_kfull(int64_t ld, int64_t k0, bool vec, bool rows_in) {
     if (!RC && vec) {
 #pragma unroll
       for (int q = 0; q < NR; ++q) {
-        const V4 x = *(gptr<const V4>)(P + (k0 + 4 * krun(q)) + (r0 + row(q)) * ld);
+        const V4 x = *(gptr<const V4>)(lp[q] + k0);
 #pragma unroll
         for (int e = 0; e < 4; ++e) v[q][e] = x[e];
       }
-      return;
+    } else {
+#pragma unroll
+      for (int q = 0; q < NR; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[q][e] = RC ? lp[q][(k0 + e) * ld] : lp[q][k0 + e];
     }
+    if (!rows_in) {
 #pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      const int64_t gr = r0 + row(q), gk = k0 + 4 * krun(q);
+      for (int q = 0; q < NR; ++q)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) v[q][e] = RC ? P[gr + (gk + e) * ld] : P[gk + e + gr * ld];
+        for (int e = 0; e < 4; ++e) v[q][e] = rok[q] ? v[q][e] : 0.0f;
     }
   }
   // x = hi + mid + lo + rest: each plane is x rounded to BF16 (nearest even,
@@ -706,15 +727,17 @@ __device__ void x3_large_tile(const GProb& P, int tm, int tn, int split, char* s
   gptr<const float> B = as_global(static_cast<const float*>(P.B));
   GRun4<BM, BK, !TA, NTH> la(tid);
   GRun4<BN, BK, TB, NTH> lb(tid);
-  // the full-tile loads: rows inside the operand; k-contiguous operands in
+  // stages inside the operand along k take load_kfull: k-contiguous operands in
   // 16-byte vectors where the base is 16-byte aligned and ld a multiple of 4
   const bool fa = m0 + BM <= P.m, fb = n0 + BN <= P.n;
   const bool va = ((reinterpret_cast<uintptr_t>(P.A) & 15) == 0) && P.lda % 4 == 0;
   const bool vb = ((reinterpret_cast<uintptr_t>(P.B) & 15) == 0) && P.ldb % 4 == 0;
+  la.lane_base(A, P.lda, m0, P.m);
+  lb.lane_base(B, P.ldb, n0, P.n);
   auto load_stage = [&](int64_t k0) {
-    if (fa && fb && k0 + BK <= ke) {
-      la.load_full(A, P.lda, m0, k0, va);
-      lb.load_full(B, P.ldb, n0, k0, vb);
+    if (k0 + BK <= ke) {
+      la.load_kfull(P.lda, k0, va, fa);
+      lb.load_kfull(P.ldb, k0, vb, fb);
     } else {
       la.load(A, P.lda, m0, k0, P.m, ke);
       lb.load(B, P.ldb, n0, k0, P.n, ke);
