@@ -109,7 +109,7 @@ def grouped_and_rrule(torch, jb, tune=True, rrule=True):
         # run: FP64 MFMA (78.6 TFLOP/s), or BF16 MFMA for 3xBF16 (2500 TFLOP/s;
         # six BF16 products per FP32 product)
         best = max(res["static_tflops"], res.get("tuned_tflops", 0.0))
-        ach, peak = (best, 78.6) if method == "f64" else (6 * best, 2500.0)
+        ach, peak = (best, 78.6) if method == "f64" else (6 * best, 2516.6)
         res["roofline"] = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
                            "frac": round(ach / peak, 4),
                            "dtype": "fp64" if method == "f64" else "bf16 (6 products per fp32 product)"}
