@@ -592,13 +592,21 @@ struct GRun4 {
       uint16_t* dst = S + r * BK + 8 * bf_chunk(r, kr >> 1) + 4 * (kr & 1);
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
-        const bf2 h0 = {(__bf16)x[0], (__bf16)x[1]}, h1 = {(__bf16)x[2], (__bf16)x[3]};
-        *reinterpret_cast<uint2*>(dst + p * R * BK) =
-            uint2{__builtin_bit_cast(uint32_t, h0), __builtin_bit_cast(uint32_t, h1)};
-        x[0] -= (float)h0[0];
-        x[1] -= (float)h0[1];
-        x[2] -= (float)h1[0];
-        x[3] -= (float)h1[1];
+        uint32_t w0 = __builtin_bit_cast(uint32_t, bf2{(__bf16)x[0], (__bf16)x[1]});
+        uint32_t w1 = __builtin_bit_cast(uint32_t, bf2{(__bf16)x[2], (__bf16)x[3]});
+        *reinterpret_cast<uint2*>(dst + p * R * BK) = uint2{w0, w1};
+        if (p < 2) {
+          // the packed BF16 values back in FP32 (low half shifted up, high half
+          // masked): one packed conversion per two elements and plane.  Written
+          // as (float) of each BF16, or with w0, w1 visible to the compiler, it
+          // converted each low element a second time on its own; the empty asm
+          // hides where w0 and w1 came from.
+          asm("" : "+v"(w0), "+v"(w1));
+          x[0] -= __uint_as_float(w0 << 16);
+          x[1] -= __uint_as_float(w0 & 0xffff0000u);
+          x[2] -= __uint_as_float(w1 << 16);
+          x[3] -= __uint_as_float(w1 & 0xffff0000u);
+        }
       }
     }
   }
