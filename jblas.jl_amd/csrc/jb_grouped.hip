@@ -590,10 +590,22 @@ struct GRun4 {
       float x[4] = {v[q][0], v[q][1], v[q][2], v[q][3]};
       const int r = row(q), kr = krun(q);
       uint16_t* dst = S + r * BK + 8 * bf_chunk(r, kr >> 1) + 4 * (kr & 1);
+      // |x| within half a BF16 ulp of FLT_MAX rounds to Inf: the first plane
+      // rounds x clamped to the largest finite BF16, which for such an x is
+      // its truncation (the residual then has at most 16 significant bits and
+      // the split stays exact, as in the single-problem split) and for every
+      // other finite x changes nothing.  An Inf or NaN x still leaves a NaN
+      // residual, so every product it enters is NaN.
+      const float BMAX = __uint_as_float(0x7F7F0000u);
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
-        uint32_t w0 = __builtin_bit_cast(uint32_t, bf2{(__bf16)x[0], (__bf16)x[1]});
-        uint32_t w1 = __builtin_bit_cast(uint32_t, bf2{(__bf16)x[2], (__bf16)x[3]});
+        float y[4] = {x[0], x[1], x[2], x[3]};
+        if (p == 0) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) y[e] = __builtin_amdgcn_fmed3f(x[e], -BMAX, BMAX);
+        }
+        uint32_t w0 = __builtin_bit_cast(uint32_t, bf2{(__bf16)y[0], (__bf16)y[1]});
+        uint32_t w1 = __builtin_bit_cast(uint32_t, bf2{(__bf16)y[2], (__bf16)y[3]});
         *reinterpret_cast<uint2*>(dst + p * R * BK) = uint2{w0, w1};
         if (p < 2) {
           // the packed BF16 values back in FP32 (low half shifted up, high half

@@ -285,6 +285,50 @@ def test_grouped_concurrent_launches_mixed_ops(gpu):
     plan.destroy()
 
 
+@pytest.mark.parametrize("tile", ["64x64", "32x32", "128x128", "128x64", "64x128"])
+def test_grouped_bf16x3_flt_max_and_nonfinite(gpu, tile, tmp_path):
+    """3xBF16 tiles split entries up to FLT_MAX exactly (the first plane
+    truncates where rounding to nearest would give Inf), so products with small
+    partners stay finite and within 2*K*2^-24 of the FP64 oracle; an Inf or NaN
+    operand entry makes the row or column of C it enters NaN (as the
+    single-problem 3xBF16 split does) and leaves the rest finite."""
+    torch = gpu
+    jb.tune_reset()
+    cfg = {v: k for k, v in jb.grouped.TILES.items()}[tile]
+    m, n, k = 100, 80, 96
+    write_table(str(tmp_path / "big3.json"), "f32_bf16x3", [(m, n, k)], cfg, 1)
+    try:
+        A = G.matrix(31, m, k, dtype=np.float32)
+        B = G.matrix(32, k, n, dtype=np.float32) * np.float32(2.0 ** -100)
+        A[:, 0] = np.float32(3.4028235e38)   # FLT_MAX
+        A[:, 1] = np.float32(-3.4e38)
+        A[:, 2] = np.float32(3.3962e38)      # nearest BF16 is Inf
+        A[5, 3] = np.float32(3.39e38)        # just below: nearest BF16 is finite
+        Ad, Bd = to_dev(torch, A), to_dev(torch, B)
+        Cd = to_dev(torch, np.zeros((m, n), dtype=np.float32))
+        desc = [GemmDesc("N", "N", 1.0, Ad, Bd, 0.0, Cd)]
+        assert jb.grouped_choices("f32_bf16x3", desc)[0][0] == tile
+        jb.gemm_grouped_("f32_bf16x3", desc)
+        torch.cuda.synchronize()
+        got = to_host(Cd).astype(np.float64)
+        ref = A.astype(np.float64) @ B.astype(np.float64)
+        absab = np.abs(A).astype(np.float64) @ np.abs(B).astype(np.float64)
+        assert np.all(np.isfinite(got)), tile
+        assert np.max(np.abs(got - ref) / absab) <= 2 * k * U32, tile
+        A[7, 5], B[9, 30] = np.inf, np.nan
+        Ad, Bd = to_dev(torch, A), to_dev(torch, B)
+        desc = [GemmDesc("N", "N", 1.0, Ad, Bd, 0.0, Cd)]
+        jb.gemm_grouped_("f32_bf16x3", desc)
+        torch.cuda.synchronize()
+        got = to_host(Cd)
+        bad = np.zeros((m, n), bool)
+        bad[7, :] = True
+        bad[:, 30] = True
+        assert np.all(np.isnan(got[bad])) and np.all(np.isfinite(got[~bad])), tile
+    finally:
+        jb.tune_reset()
+
+
 def test_grouped_bf16x3_concurrent_launches(gpu, tmp_path):
     """3xBF16 group whose problems need every persistent kernel at once: a
     tuning table puts one size bucket each on 128x128 (eight-wave workgroups,
