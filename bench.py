@@ -106,7 +106,7 @@ def grouped_and_rrule(torch, jb, tune=True, rrule=True):
             res[f"{phase}_ms"] = round(ms, 4)
             res[f"{phase}_tflops"] = round(flops / ms / 1e9, 3)
         # the grouped kernels against the matrix-core peak of the arithmetic they
-        # run: FP64 MFMA (78.6 TFLOP/s), or BF16 MFMA for 3xBF16 (2500 TFLOP/s;
+        # run: FP64 MFMA (78.6 TFLOP/s), or BF16 MFMA for 3xBF16 (2516.6 TFLOP/s;
         # six BF16 products per FP32 product)
         best = max(res["static_tflops"], res.get("tuned_tflops", 0.0))
         ach, peak = (best, 78.6) if method == "f64" else (6 * best, 2516.6)

@@ -113,7 +113,7 @@ def test_noalloc_guard_bands(gpu):
 def test_grouped_guard_bands(gpu, method, dtype, tmp_path):
     """Mixed shapes in one launch: small-path problems (m, n, k <= 64), MFMA
     tiles with fringes, and tiles and split-K forced through a tuning table
-    (the 128x128 FP64 tile included)."""
+    (the 128-row/column FP64 and 3xBF16 tiles included)."""
     torch = gpu
     shapes = [(17, 9, 5), (64, 64, 64), (130, 70, 45), (33, 200, 700), (1, 1, 1), (96, 40, 1500)]
     guards, descs = [], []
@@ -129,14 +129,14 @@ def test_grouped_guard_bands(gpu, method, dtype, tmp_path):
     for i, (g, _, _) in enumerate(guards):
         g.check(f"grouped {method} problem {i} {shapes[i]}")
     # the same group with tile and split-K forced on every bucket (tuning
-    # table): 32x32 tiles with 4 splits; for FP64 also the 128x128 tile with
-    # one split and with 2, and the 128x64 and 64x128 tiles
+    # table): 32x32 tiles with 4 splits; for FP64 and 3xBF16 also the 128x128
+    # tile with one split and with 2, and the 128x64 and 64x128 tiles
     path = tmp_path / "tune.json"
     dev = jb.device_info()
     mid = jb.grouped.method_id(method)
     # the table's bucket key: method, floor(log2(floor(sqrt(m n)))), floor(log2 k)
     keys = [f"m{mid}|mn{int(np.sqrt(float(m) * n)).bit_length() - 1}|k{k.bit_length() - 1}" for (m, n, k) in shapes]
-    forced = [(1, 4)] + ([(3, 1), (3, 2), (4, 2), (5, 1)] if method == "f64" else [])
+    forced = [(1, 4)] + ([(3, 1), (3, 2), (4, 2), (5, 1)] if method in ("f64", "f32_bf16x3") else [])
     try:
         for cfg, splitk in forced:
             entries = ",\n".join(f'    {{"key": "{dev["name"]}|{dev["arch"]}|{key}", "cfg": {cfg}, '
