@@ -49,9 +49,11 @@ hipError_t launch_dgemm_ozaki(const GemmArgs& g, int s, void* ws, int* sticky, h
 // The same with INT8 slices (v_mfma_i32_32x32x32_i8, exact int32 level sums).
 size_t ozaki_i8_workspace_bytes(int64_t m, int64_t n, int64_t k, int s);
 hipError_t launch_dgemm_ozaki_i8(const GemmArgs& g, int s, void* ws, int* sticky, hipStream_t st);
-// Byte offset of the call's non-finite flag in the workspace (s = 3, i8 = false
-// for the 3xBF16 SGEMM).
+// Byte offset of the call's non-finite flag in the workspace of the emulated
+// DGEMM (BF16 or INT8 slices) and of the 3xBF16 SGEMM (whose planes have their
+// own layout).
 size_t emulated_flag_offset(int64_t m, int64_t n, int64_t k, int s, bool i8);
+size_t bf16x3_flag_offset(int64_t m, int64_t n, int64_t k);
 
 // ---- FP32 -----------------------------------------------------------------
 hipError_t launch_sgemm_mfma(const GemmArgs& g, hipStream_t s);

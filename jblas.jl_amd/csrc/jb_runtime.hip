@@ -822,7 +822,7 @@ static int sgemm_dev_locked(const GemmArgs& g, int method, int nprod, hipStream_
   if (rc) return rc;
   e = launch_sgemm_bf16x3(g, nprod, ws, rt().sticky, s);
   if (e == hipSuccess && host_flag)
-    e = hipMemcpyAsync(host_flag, static_cast<char*>(ws) + emulated_flag_offset(g.m, g.n, g.k, 3, false), sizeof(int),
+    e = hipMemcpyAsync(host_flag, static_cast<char*>(ws) + bf16x3_flag_offset(g.m, g.n, g.k), sizeof(int),
                        hipMemcpyDeviceToHost, s);
   rc = ws_release(s);
   if (e != hipSuccess) return hip_fail(e, "sgemm_bf16x3 launch");

@@ -69,7 +69,10 @@ constexpr int QPAD_K = 64;          // k padding of the planes (split tile width
 #define OZ_GROUP 2  // tile-rows per group of the tile order (measured best of 2, 4, 8, 16)
 #endif
 #ifndef X3_WB
-#define X3_WB 2  // 3xBF16: 4 waves of 64 x 32
+#define X3_WB 2  // 3xBF16: waves of 64 x 32
+#endif
+#ifndef X3_BN
+#define X3_BN 128  // 3xBF16 block tile 128 x 128 (8 waves)
 #endif
 
 // k-chunk of 0-based level L (L + 1 slice pairs) over which its FP32 sum of
@@ -487,14 +490,16 @@ __device__ __forceinline__ void block_barrier() {
   asm volatile("" ::: "memory");
 }
 
-// WB = 32x32 MFMA blocks per wave along m: WB = 1 -> 8 waves (2 per SIMD) of
-// 32 x 32; WB = 2 -> 4 waves (1 per SIMD) of 64 x 32.  Both tile 128 x 64.
-template <int MODE, int S, int LMAX, int WB>
-__global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p) {
-  constexpr int NWAVES = 8 / WB, WM = 4 / WB;     // waves, waves along m
-  constexpr int PLANE_B = (QBM + QBN) * QBK * 2;  // bytes of one plane in one stage (6 KB)
+// WB = 32x32 MFMA blocks per wave along m: WB = 1 -> waves of 32 x 32, WB = 2
+// -> waves of 64 x 32; block tile 128 x BN (BN = 64: 8 or 4 waves; BN = 128,
+// the 3xBF16 mode: 8 waves of 64 x 32, which fetch half the B bytes per
+// product of two 128 x 64 tiles).
+template <int MODE, int S, int LMAX, int WB, int BN = QBN>
+__global__ __launch_bounds__((64 * (QBM / (32 * WB)) * (BN / 32)), 1) void slice_pair_gemm_kernel(PairArgs p) {
+  constexpr int WM = QBM / (32 * WB), NWAVES = WM * (BN / 32);  // waves along m, waves
+  constexpr int PLANE_B = (QBM + BN) * QBK * 2;  // bytes of one plane in one stage (6 KB for BN = 64)
   constexpr int STAGE_B = S * PLANE_B;
-  constexpr int NWL = 6 * S;                      // 1 KB wave-loads per stage
+  constexpr int NWL = (QBM / 32 + BN / 32) * S;  // 1 KB wave-loads per stage
   constexpr int WL_MIN = NWL / NWAVES;            // wave-loads per stage of waves >= NWL % NWAVES
   constexpr int WL_MAX = (NWL + NWAVES - 1) / NWAVES;  // ... of waves < NWL % NWAVES
   constexpr int NL = (MODE == 1) ? 2 : S;         // accumulator sets (levels)
@@ -518,7 +523,7 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   const int tile = xcd_remap(blockIdx.x, nblocks, 8);
   int tm, tn;
   tile_coords(tile, p.tiles_m, p.tiles_n, OZ_GROUP, tm, tn);
-  const int64_t m0 = (int64_t)tm * QBM, n0 = (int64_t)tn * QBN;
+  const int64_t m0 = (int64_t)tm * QBM, n0 = (int64_t)tn * BN;
   const int tid = threadIdx.x, lane = tid % 64;
   // wave-uniform (SGPR) so that the DMA issue below is scalar code
   const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
@@ -538,9 +543,9 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
   // first put a 64-bit vector add (and selects) in front of every load; this
   // form took the s = 6 kernel from 19.9 to 19.0 ms at 8192^3.
   const uint16_t* gA = p.pa + (int64_t)tm * QBM * p.Kpad;
-  const uint16_t* gB = p.pb + (int64_t)tn * QBN * p.Kpad;
+  const uint16_t* gB = p.pb + (int64_t)tn * BN * p.Kpad;
   // Piece i of this wave's share of stage ks (pieces j = wave + NWAVES*i; A:
-  // plane t, 4 pieces of 1 KB; B: plane t, 2 pieces) -> LDS slot `slot`.
+  // plane t, 4 pieces of 1 KB; B: plane t, BN/32 pieces) -> LDS slot `slot`.
   auto issue_piece = [&](int ks, int slot, int i) JB_INLINE {
     const int j = wave + NWAVES * i;  // uniform
     if (j >= NWL) return;
@@ -551,8 +556,8 @@ __global__ __launch_bounds__(512 / WB, 1) void slice_pair_gemm_kernel(PairArgs p
       src = gA + t * p.strideA + (int64_t)ks * (QBM * QBK) + q * 512;
       loff = t * PLANE_B + q * 1024;
     } else {
-      const int jj = j - 4 * S, t = jj / 2, q = jj % 2;
-      src = gB + t * p.strideB + (int64_t)ks * (QBN * QBK) + q * 512;
+      const int jj = j - 4 * S, t = jj / (BN / 32), q = jj % (BN / 32);
+      src = gB + t * p.strideB + (int64_t)ks * (BN * QBK) + q * 512;
       loff = t * PLANE_B + QBM * QBK * 2 + q * 1024;
     }
     __builtin_amdgcn_global_load_lds((const void*)(src + (uint32_t)dma_off), (JB_LDS void*)(ring + slot * STAGE_B + loff),
@@ -775,11 +780,12 @@ struct Layout {
   size_t offA, offB, offMA, offMB, offEA, offEB, offFlag, total;
 };
 
-// elt: bytes per digit (2: BF16 planes, 1: INT8 planes).
-Layout make_layout(int64_t m, int64_t n, int64_t k, int s, int elt = 2) {
+// elt: bytes per digit (2: BF16 planes, 1: INT8 planes); bn: block tile width
+// of the slice-pair kernel (op(B) row block of the planes).
+Layout make_layout(int64_t m, int64_t n, int64_t k, int s, int elt = 2, int bn = QBN) {
   Layout L;
   L.Mpad = rup(std::max<int64_t>(m, 1), QBM);
-  L.Npad = rup(std::max<int64_t>(n, 1), QBN);
+  L.Npad = rup(std::max<int64_t>(n, 1), bn);
   L.Kpad = rup(std::max<int64_t>(k, 1), QPAD_K);
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   size_t off = 0;
@@ -832,7 +838,7 @@ void clear_call_flag(int* call_flag, hipStream_t st) {
 // elt as in make_layout: the kernel addresses planes in 16-bit units, so an
 // INT8 plane of Kpad k is a BF16-shaped plane of Kpad/2.
 PairArgs pair_args(const GemmArgs& g, const Layout& L, const uint16_t* pA, const uint16_t* pB, const int* eA,
-                   const int* eB, int elt = 2) {
+                   const int* eB, int elt = 2, int bn = QBN) {
   PairArgs p;
   p.pa = pA;
   p.pb = pB;
@@ -848,7 +854,7 @@ PairArgs pair_args(const GemmArgs& g, const Layout& L, const uint16_t* pA, const
   p.C = g.C;
   p.ldc = g.ldc;
   p.tiles_m = (int)(L.Mpad / QBM);
-  p.tiles_n = (int)(L.Npad / QBN);
+  p.tiles_n = (int)(L.Npad / bn);
   return p;
 }
 
@@ -963,10 +969,11 @@ hipError_t launch_dgemm_ozaki(const GemmArgs& g, int s, void* ws, int* sticky, h
   }
 }
 
-size_t bf16x3_workspace_bytes(int64_t m, int64_t n, int64_t k) { return make_layout(m, n, k, 3).total; }
+size_t bf16x3_workspace_bytes(int64_t m, int64_t n, int64_t k) { return make_layout(m, n, k, 3, 2, X3_BN).total; }
+size_t bf16x3_flag_offset(int64_t m, int64_t n, int64_t k) { return make_layout(m, n, k, 3, 2, X3_BN).offFlag; }
 
 hipError_t launch_sgemm_bf16x3(const GemmArgs& g, int nprod, void* ws, int* sticky, hipStream_t st) {
-  const Layout L = make_layout(g.m, g.n, g.k, 3);
+  const Layout L = make_layout(g.m, g.n, g.k, 3, 2, X3_BN);
   char* w = static_cast<char*>(ws);
   auto* pA = reinterpret_cast<uint16_t*>(w + L.offA);
   auto* pB = reinterpret_cast<uint16_t*>(w + L.offB);
@@ -979,16 +986,17 @@ hipError_t launch_sgemm_bf16x3(const GemmArgs& g, int nprod, void* ws, int* stic
                      static_cast<const float*>(g.A), sr, sk, g.m, g.k, pA, L.Mpad * L.Kpad, L.Kpad, QBM, flag, sticky);
   strides_b(g.tb, g.ldb, sr, sk);
   hipLaunchKernelGGL(split_f32_kernel, dim3((unsigned)(L.Npad / 64), (unsigned)(L.Kpad / 64)), dim3(256), 0, st,
-                     static_cast<const float*>(g.B), sr, sk, g.n, g.k, pB, L.Npad * L.Kpad, L.Kpad, QBN, flag, sticky);
+                     static_cast<const float*>(g.B), sr, sk, g.n, g.k, pB, L.Npad * L.Kpad, L.Kpad, X3_BN, flag, sticky);
   timing_end(st, "bf16x3_split");
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const PairArgs p = pair_args(g, L, pA, pB, nullptr, nullptr);
+  const PairArgs p = pair_args(g, L, pA, pB, nullptr, nullptr, 2, X3_BN);
   const dim3 grid(p.tiles_m * p.tiles_n);
+  const dim3 block(64 * (QBM / (32 * X3_WB)) * (X3_BN / 32));
   timing_begin(st, "bf16x3_gemm");
-  if (nprod == 3) hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 3, X3_WB>), grid, dim3(512 / X3_WB), 0, st, p);
-  else if (nprod == 6) hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 4, X3_WB>), grid, dim3(512 / X3_WB), 0, st, p);
-  else hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 6, X3_WB>), grid, dim3(512 / X3_WB), 0, st, p);
+  if (nprod == 3) hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 3, X3_WB, X3_BN>), grid, block, 0, st, p);
+  else if (nprod == 6) hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 4, X3_WB, X3_BN>), grid, block, 0, st, p);
+  else hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 6, X3_WB, X3_BN>), grid, block, 0, st, p);
   timing_end(st, "bf16x3_gemm");
   return hipGetLastError();
 }
