@@ -71,9 +71,6 @@ constexpr int QPAD_K = 64;          // k padding of the planes (split tile width
 #ifndef X3_WB
 #define X3_WB 2  // 3xBF16: waves of 64 x 32
 #endif
-#ifndef X3_BN
-#define X3_BN 128  // 3xBF16 block tile 128 x 128 (8 waves)
-#endif
 
 // k-chunk of 0-based level L (L + 1 slice pairs) over which its FP32 sum of
 // products of digits |d| <= 256 stays within 2^24, i.e. exact.
@@ -491,9 +488,8 @@ __device__ __forceinline__ void block_barrier() {
 }
 
 // WB = 32x32 MFMA blocks per wave along m: WB = 1 -> waves of 32 x 32, WB = 2
-// -> waves of 64 x 32; block tile 128 x BN (BN = 64: 8 or 4 waves; BN = 128,
-// the 3xBF16 mode: 8 waves of 64 x 32, which fetch half the B bytes per
-// product of two 128 x 64 tiles).
+// -> waves of 64 x 32; block tile 128 x BN (BN = 64: 8 or 4 waves; the 3xBF16
+// mode: BN = 128 or 256, 8 or 16 waves of 64 x 32, fewer B bytes per product).
 template <int MODE, int S, int LMAX, int WB, int BN = QBN>
 __global__ __launch_bounds__((64 * (QBM / (32 * WB)) * (BN / 32)), 1) void slice_pair_gemm_kernel(PairArgs p) {
   constexpr int WM = QBM / (32 * WB), NWAVES = WM * (BN / 32);  // waves along m, waves
@@ -969,11 +965,34 @@ hipError_t launch_dgemm_ozaki(const GemmArgs& g, int s, void* ws, int* sticky, h
   }
 }
 
-size_t bf16x3_workspace_bytes(int64_t m, int64_t n, int64_t k) { return make_layout(m, n, k, 3, 2, X3_BN).total; }
-size_t bf16x3_flag_offset(int64_t m, int64_t n, int64_t k) { return make_layout(m, n, k, 3, 2, X3_BN).offFlag; }
+// Block width of the 3xBF16 kernel (and row block of its op(B) planes): 256
+// (sixteen waves, one workgroup per CU, a quarter fewer plane bytes per product
+// than 128) where that grid holds at least two tiles per CU, else 128 (eight
+// waves, two workgroups per CU).  At 8192^3: 277 -> 305 TF/s with 3 products,
+// 197 -> 207 with 6; 4096 x 2048 x 4096 with 6: 165 -> 171; 2048^3 (128 tiles
+// of 128 x 256): 149 -> 119, hence the 128-wide tile there.
+int x3_bn(int64_t m, int64_t n) {
+  const int64_t t256 = ((m + QBM - 1) / QBM) * ((n + 255) / 256);
+  return t256 >= 2 * (int64_t)std::max(runtime_compute_units(), 1) ? 256 : 128;
+}
+
+size_t bf16x3_workspace_bytes(int64_t m, int64_t n, int64_t k) {
+  return make_layout(m, n, k, 3, 2, x3_bn(m, n)).total;
+}
+size_t bf16x3_flag_offset(int64_t m, int64_t n, int64_t k) { return make_layout(m, n, k, 3, 2, x3_bn(m, n)).offFlag; }
+
+template <int BN>
+void launch_x3_pairs(const PairArgs& p, int nprod, hipStream_t st) {
+  const dim3 grid(p.tiles_m * p.tiles_n);
+  const dim3 block(64 * (QBM / (32 * X3_WB)) * (BN / 32));
+  if (nprod == 3) hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 3, X3_WB, BN>), grid, block, 0, st, p);
+  else if (nprod == 6) hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 4, X3_WB, BN>), grid, block, 0, st, p);
+  else hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 6, X3_WB, BN>), grid, block, 0, st, p);
+}
 
 hipError_t launch_sgemm_bf16x3(const GemmArgs& g, int nprod, void* ws, int* sticky, hipStream_t st) {
-  const Layout L = make_layout(g.m, g.n, g.k, 3, 2, X3_BN);
+  const int bn = x3_bn(g.m, g.n);
+  const Layout L = make_layout(g.m, g.n, g.k, 3, 2, bn);
   char* w = static_cast<char*>(ws);
   auto* pA = reinterpret_cast<uint16_t*>(w + L.offA);
   auto* pB = reinterpret_cast<uint16_t*>(w + L.offB);
@@ -986,17 +1005,14 @@ hipError_t launch_sgemm_bf16x3(const GemmArgs& g, int nprod, void* ws, int* stic
                      static_cast<const float*>(g.A), sr, sk, g.m, g.k, pA, L.Mpad * L.Kpad, L.Kpad, QBM, flag, sticky);
   strides_b(g.tb, g.ldb, sr, sk);
   hipLaunchKernelGGL(split_f32_kernel, dim3((unsigned)(L.Npad / 64), (unsigned)(L.Kpad / 64)), dim3(256), 0, st,
-                     static_cast<const float*>(g.B), sr, sk, g.n, g.k, pB, L.Npad * L.Kpad, L.Kpad, X3_BN, flag, sticky);
+                     static_cast<const float*>(g.B), sr, sk, g.n, g.k, pB, L.Npad * L.Kpad, L.Kpad, bn, flag, sticky);
   timing_end(st, "bf16x3_split");
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const PairArgs p = pair_args(g, L, pA, pB, nullptr, nullptr, 2, X3_BN);
-  const dim3 grid(p.tiles_m * p.tiles_n);
-  const dim3 block(64 * (QBM / (32 * X3_WB)) * (X3_BN / 32));
+  const PairArgs p = pair_args(g, L, pA, pB, nullptr, nullptr, 2, bn);
   timing_begin(st, "bf16x3_gemm");
-  if (nprod == 3) hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 3, X3_WB, X3_BN>), grid, block, 0, st, p);
-  else if (nprod == 6) hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 4, X3_WB, X3_BN>), grid, block, 0, st, p);
-  else hipLaunchKernelGGL((slice_pair_gemm_kernel<1, 3, 6, X3_WB, X3_BN>), grid, block, 0, st, p);
+  if (bn == 256) launch_x3_pairs<256>(p, nprod, st);
+  else launch_x3_pairs<128>(p, nprod, st);
   timing_end(st, "bf16x3_gemm");
   return hipGetLastError();
 }
