@@ -309,6 +309,28 @@ def test_sgemm_bf16x3_tolerance(gpu, ta, tb):
     assert errs[3] > errs[6]
 
 
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+def test_sgemm_bf16x3_wide_block_fringes(gpu, ta, tb):
+    """A problem large enough for the 128x256 3xBF16 block on a 256-CU device
+    (32 x 16 tiles) whose m and n are not multiples of the block (partial row
+    blocks of the op(B) planes): the first and last 64 rows and columns within
+    2*K*2^-24 componentwise of an FP64 product of the same FP32 operands."""
+    m, n, k = 4000, 3900, 200
+    A = G.matrix(45, m, k, dtype=np.float32)
+    B = G.matrix(46, k, n, dtype=np.float32)
+    C0 = G.matrix(47, m, n, dtype=np.float32)
+    Cd = to_dev(gpu, C0)
+    jb.gemm_(ta, tb, 1.5, to_dev(gpu, opnd(A, ta)), to_dev(gpu, opnd(B, tb)), 0.5, Cd, method="bf16x3", nprod=6)
+    got = to_host(Cd).astype(float)
+    rows = np.r_[0:64, m - 64:m]
+    cols = np.r_[0:64, n - 64:n]
+    A64, B64 = A[rows].astype(float), B[:, cols].astype(float)
+    ref = 1.5 * (A64 @ B64) + 0.5 * C0[np.ix_(rows, cols)].astype(float)
+    bound = 1.5 * (np.abs(A64) @ np.abs(B64)) + 0.5 * np.abs(C0[np.ix_(rows, cols)]).astype(float)
+    err = np.max(np.abs(got[np.ix_(rows, cols)] - ref) / bound)
+    assert err <= 2 * k * 2.0 ** -24, (ta, tb, err)
+
+
 def test_sgemm_bf16x3_c3_8192(gpu):
     """Config: Float32 inputs via the 3xBF16 split at M=N=K=8192, a_ij = u*2^e
     (e uniform in [-12, 12]) generated in HBM (bit-identical to the host
