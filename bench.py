@@ -424,7 +424,9 @@ def main():
                    "kernel_ms": round(k_ms / max(k_n, 1), 4), "out": C32.clone() if not args.skip_cpu else None}
             if meth == "bf16x3":
                 hw = nprod * flops / (k_ms / max(k_n, 1) * 1e-3) / 1e12
-                row["roofline"] = {"kernel": f"slice_pair_gemm_kernel<1,3,{ {3: 3, 6: 4, 9: 6}[nprod] },2>",
+                # block width the library picks: 256 where that grid holds two tiles per CU
+                bn = 256 if -(-n // 128) * -(-n // 256) >= 2 * info["compute_units"] else 128
+                row["roofline"] = {"kernel": f"slice_pair_gemm_kernel<1,3,{ {3: 3, 6: 4, 9: 6}[nprod] },2,{bn}>",
                                    "bound": "mfma", "achieved": round(hw, 2),
                                    "peak": round(info["peak_bf16_tflops"], 1), "unit": "TFLOP/s",
                                    "frac": round(hw / info["peak_bf16_tflops"], 4)}
