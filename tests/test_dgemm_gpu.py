@@ -405,3 +405,34 @@ def test_alloc_failure_is_reported(gpu):
         del os.environ["JB_FAULT_ALLOC"]
     jb.gemm_("N", "N", 1.0, A, B, 0.0, C)
     assert np.isfinite(C).all()
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_native_nonfinite_matches_recurrence(gpu, ta, tb):
+    """Inf, -Inf and NaN operand entries and products that overflow: every
+    element of the FP64 MFMA result equals the reference recurrence's bit for
+    bit where that is a number or an infinity, and is NaN where it is NaN
+    (IEEE fma on both sides; NaN payloads are not compared), on full and
+    fringe tiles of the 128-, 64- and 32-row variants."""
+    m, n, k = 150, 90, 120
+    A = G.matrix(71, m, k)
+    B = G.matrix(72, k, n)
+    C0 = G.matrix(73, m, n)
+    A[3, 7], A[40, 0], A[149, 119] = np.inf, -np.inf, np.nan
+    B[5, 11], B[119, 89] = np.nan, -np.inf
+    A[60, :] *= 1e300  # row 60 of the products overflows against B's larger entries
+    B[:, 20] *= 1e10
+    ref = O.gemm_recurrence(ta, tb, 1.5, opnd(A, ta), opnd(B, tb), 0.5, C0)
+    for tile in ("128", "64", "32"):
+        os.environ["JB_GEMM_TILE"] = tile
+        try:
+            C = to_dev(gpu, C0)
+            jb.gemm_(ta, tb, 1.5, to_dev(gpu, opnd(A, ta)), to_dev(gpu, opnd(B, tb)), 0.5, C, method="native")
+            gpu.cuda.synchronize()
+            got = to_host(C)
+        finally:
+            del os.environ["JB_GEMM_TILE"]
+        nan_ref, nan_got = np.isnan(ref), np.isnan(got)
+        assert np.array_equal(nan_ref, nan_got), (tile, ta, tb, np.argwhere(nan_ref != nan_got)[:5])
+        assert np.array_equal(got[~nan_ref].view(np.int64), ref[~nan_ref].view(np.int64)), (tile, ta, tb)
+        assert np.isinf(ref).any() and nan_ref.any()
