@@ -782,3 +782,38 @@ def test_torch_autograd_binding(gpu):
     (Ac @ Bc * W.cpu()).sum().backward()
     assert torch.allclose(A.grad.cpu(), Ac.grad, rtol=0, atol=1e-12)
     assert torch.allclose(B.grad.cpu(), Bc.grad, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("tile", ["128x128", "128x64", "64x128", "64x64", "32x32", "small"])
+def test_grouped_f64_nonfinite_matches_recurrence(gpu, tile, tmp_path):
+    """Inf, -Inf and NaN operand entries and overflowing products in a grouped
+    FP64 launch, the tile forced through a tuning table, two splits on the MFMA
+    tiles: bit-exact against the chunked recurrence where the result is a
+    number or an infinity, NaN where it is NaN (payloads not compared)."""
+    torch = gpu
+    jb.tune_reset()
+    m, n, k = (60, 50, 64) if tile == "small" else (150, 90, 120)
+    cfg = {v: c for c, v in jb.grouped.TILES.items()}[tile]
+    write_table(str(tmp_path / "nf.json"), "f64", [(m, n, k)], cfg, 1 if tile == "small" else 2)
+    try:
+        A = G.matrix(81, m, k)
+        B = G.matrix(82, k, n)
+        C0 = G.matrix(83, m, n)
+        A[3, 7], A[40, 0], A[m - 1, k - 1] = np.inf, -np.inf, np.nan
+        B[5, 11], B[k - 1, n - 1] = np.nan, -np.inf
+        A[30, :] *= 1e300
+        B[:, 20] *= 1e10
+        Ad, Bd, Cd = dev_copy(torch, A), dev_copy(torch, B), dev_copy(torch, C0)
+        desc = [GemmDesc("N", "N", 1.5, Ad, Bd, 0.5, Cd)]
+        ch = jb.grouped_choices("f64", desc)[0]
+        assert ch[0] == tile, ch
+        jb.gemm_grouped_("f64", desc)
+        torch.cuda.synchronize()
+        got = to_host(Cd)
+        ref = O.gemm_recurrence_chunked("N", "N", 1.5, A, B, 0.5, C0, ch[2])
+        nan_ref, nan_got = np.isnan(ref), np.isnan(got)
+        assert np.array_equal(nan_ref, nan_got), (tile, np.argwhere(nan_ref != nan_got)[:5])
+        assert np.array_equal(got[~nan_ref].view(np.int64), ref[~nan_ref].view(np.int64)), tile
+        assert np.isinf(ref).any() and nan_ref.any()
+    finally:
+        jb.tune_reset()
