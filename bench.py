@@ -574,7 +574,7 @@ def main():
         hw = pairs8 * flops
         ach_i = hw / (i8["gemm_kernel_ms"] * 1e-3) / 1e12
         peak_i8 = info["peak_bf16_tflops"] * 2.0  # int8 MFMA: 8192 ops/clk/CU
-        line["emulated_int8"]["roofline"] = {"kernel": f"slice_pair_gemm_kernel<2,{s8},{s8 + 1},1>", "bound": "mfma",
+        line["emulated_int8"]["roofline"] = {"kernel": f"slice_pair_gemm_kernel<2,{s8},{s8 + 1},1,64>", "bound": "mfma",
                                              "achieved": round(ach_i, 2), "peak": round(peak_i8, 1),
                                              "unit": "TOP/s", "frac": round(ach_i / peak_i8, 4),
                                              "traffic": profiled_traffic(f"slice_pair_gemm_i8_s{s8}_n{n}"),
@@ -602,7 +602,7 @@ def main():
         hw_flops = pairs * flops
         ach = hw_flops / (oz["gemm_kernel_ms"] * 1e-3) / 1e12
         traffic = profiled_traffic(f"slice_pair_gemm_s{s}_n{n}")
-        line["roofline"] = {"kernel": f"slice_pair_gemm_kernel<0,{s},{s + 1},1>", "bound": "mfma",
+        line["roofline"] = {"kernel": f"slice_pair_gemm_kernel<0,{s},{s + 1},1,64>", "bound": "mfma",
                             "achieved": round(ach, 2), "peak": round(info["peak_bf16_tflops"], 1),
                             "unit": "TFLOP/s", "frac": round(ach / info["peak_bf16_tflops"], 4),
                             "traffic": traffic,
